@@ -1,0 +1,200 @@
+"""Pins of the oracle stepper (oracle/stepper.c via oracle/starmap_oracle.py)
+against models that do not restate its code:
+
+* the dense one-step operator from Kronecker products (tests/dense_ops.py),
+  for both boundary conditions, 2D and 3D, with filters and anisotropic
+  scattering;
+* the Fourier amplification matrix of a single mode on periodic grids;
+* conservation of the zeroth-moment mass without absorption, and its exact
+  per-step decay factor with homogeneous absorption;
+* P_N == SP_N for the setup of starmap_2d_ex_boxes.m (machine precision, as
+  that example states);
+* the closed form exp(-dt f_eff (k/N)^p) of the exponential filter;
+* the linear-in-time output interpolation at a t_plot between two steps.
+"""
+import numpy as np
+import pytest
+
+from oracle import starmap_oracle as so
+from oracle.closure import closure_pn, closure_spn
+from starmap_amd import problems as P
+from tests.dense_ops import dense_step_operator, fourier_step_matrix, two_colour
+
+
+def _rand_ic(dim, seed=0):
+    if dim == 2:
+        def ic(x, y, k):
+            return np.random.default_rng(seed + 1000 * k).normal(size=np.shape(x))
+    else:
+        def ic(x, y, z, k):
+            return np.random.default_rng(seed + 1000 * k).normal(size=np.shape(x))
+    return ic
+
+
+def _state(out):
+    return np.concatenate([np.asarray(o["U"]).ravel(order="F") for o in out])
+
+
+def _e(x):
+    return np.expm1(x) / x if x != 0 else 1.0
+
+
+@pytest.mark.parametrize("dim,N,clo,bc,n", [
+    (2, 3, "P", 1, (5, 4)),
+    (2, 3, "P", 0, (5, 4)),
+    (2, 4, "SP", 1, (4, 3)),
+    (3, 1, "P", 1, (3, 4, 2)),
+    (3, 2, "P", 0, (3, 3, 4)),
+])
+@pytest.mark.parametrize("filt", [False, True])
+def test_dense_operator(dim, N, clo, bc, n, filt):
+    sa, ss = 0.3, 0.7
+    ax = [0.0, 1.0, 0.0, 1.3, 0.0, 0.8][:2 * dim]
+    prob = {"name": "dense", "closure": clo, "n_mom": N, "ic": _rand_ic(dim, 7),
+            "ax": ax, "n": list(n), "bc": [bc] * dim, "t_plot": [0.25], "tfinal": 0.25,
+            "cn": 0.9, "mom_output": 1, "output": None}
+    if dim == 2:
+        prob["sigma_a"] = lambda x, y: sa
+        prob["sigma_s0"] = lambda x, y: ss
+    else:
+        prob["sigma_a"] = lambda x, y, z: sa
+        prob["sigma_s0"] = lambda x, y, z: ss
+        prob["sigma_sm"] = lambda x, y, z, m: 0.5 ** m
+    if filt:
+        prob["filterfunction"] = P.exp_filter(2, 3.0)
+        prob["f_position"] = "substep"
+    par = so.starmap_init(prob)
+    u0 = _state(so.starmap_solver(dict(par, tfinal=0.0, t_plot=[]), verbose=False))
+    out, p2 = so.starmap_solver(par, verbose=False, return_state=True)
+    steps, dt = p2["_oracle"]["step_count"], p2["_oracle"]["dt"]
+    assert steps >= 2
+    Mx, My, Mz = par["Mx"], par["My"], par["Mz"]
+    nsys = Mz.shape[0]
+    mo = np.atleast_1d(par["mom_order"])
+    nn = np.array(par["n"])
+    h = (np.array(par["ax"])[[1, 3, 5]] - np.array(par["ax"])[[0, 2, 4]]) / nn
+    s = np.zeros(nsys)
+    for j in range(nsys):
+        if j == 0:
+            s[j] = sa
+        elif dim == 3:
+            s[j] = sa + ss - 0.5 ** mo[j]
+        else:
+            s[j] = sa + ss
+    E = np.array([_e(-v * dt / 2) for v in s])
+    F1 = F2 = None
+    if filt:
+        fo = mo
+        eps = np.finfo(float).eps
+        f = np.array([np.exp(-dt * 3.0 * (fo[j] / N) ** 2) for j in range(nsys)])
+        F1 = F2 = f
+    A, off, gt = dense_step_operator(Mx, My, Mz, nn, par["bc"], h, dt, s, E, F1, F2)
+    u = u0.copy()
+    for _ in range(steps):
+        u = A @ u
+    got = _state(out)
+    assert got.shape == u.shape
+    assert np.linalg.norm(got - u) <= 1e-12 * np.linalg.norm(u)
+
+
+@pytest.mark.parametrize("dim,N,n", [(2, 5, 16), (2, 4, 12), (3, 3, 8)])
+def test_fourier_mode(dim, N, n):
+    """A single Fourier mode on a periodic grid evolves by the amplification
+    matrix; each component is sampled at its own staggered positions."""
+    L = 2.0
+    kvec = np.array([0, 2, -1]) if dim == 2 else np.array([1, 2, -1])
+    xi = 2 * np.pi * kvec / L
+    rng = np.random.default_rng(3)
+    _, Mx, My, Mz = closure_pn(N, dim)
+    nsys = Mz.shape[0]
+    c0 = rng.normal(size=nsys) + 1j * rng.normal(size=nsys)
+    sa, ss = 0.2, 1.0
+    if dim == 2:
+        def ic(x, y, k):
+            return np.real(c0[k - 1] * np.exp(1j * (xi[1] * x + xi[2] * y)))
+        prob = {"ic": ic, "sigma_a": lambda x, y: sa, "sigma_s0": lambda x, y: ss,
+                "ax": [-1, 1, -1, 1], "n": [n, n], "bc": [0, 0]}
+    else:
+        def ic(x, y, z, k):
+            return np.real(c0[k - 1] * np.exp(1j * (xi[0] * x + xi[1] * y + xi[2] * z)))
+        prob = {"ic": ic, "sigma_a": lambda x, y, z: sa,
+                "sigma_s0": lambda x, y, z: ss,
+                "ax": [-1, 1, -1, 1, -1, 1], "n": [n, n, n], "bc": [0, 0, 0]}
+    prob.update({"name": "fourier", "closure": "P", "n_mom": N, "t_plot": [0.3],
+                 "tfinal": 0.3, "cn": 0.95, "mom_output": 1, "output": None})
+    par = so.starmap_init(prob)
+    out, p2 = so.starmap_solver(par, verbose=False, return_state=True)
+    steps, dt = p2["_oracle"]["step_count"], p2["_oracle"]["dt"]
+    h = np.array([2.0 / (n if dim == 3 else 1), 2.0 / n, 2.0 / n])
+    if dim == 2:
+        h[0] = 2.0
+    s = np.array([sa] + [sa + ss] * (nsys - 1))
+    E = np.array([_e(-v * dt / 2) for v in s])
+    G, gt = fourier_step_matrix(Mx, My, Mz, xi, h, dt, s, E)
+    c = np.linalg.matrix_power(G, steps) @ c0
+    for j in range(nsys):
+        o = out[j]
+        if dim == 2:
+            X, Y = np.meshgrid(o["x"], o["y"], indexing="ij")
+            want = np.real(c[j] * np.exp(1j * (xi[1] * X + xi[2] * Y)))
+        else:
+            X, Y, Z = np.meshgrid(o["x"], o["y"], o["z"], indexing="ij")
+            want = np.real(c[j] * np.exp(1j * (xi[0] * X + xi[1] * Y + xi[2] * Z)))
+        assert np.max(np.abs(o["U"] - want)) < 1e-12 * (1 + np.max(np.abs(want)))
+
+
+def test_mass_conservation_and_exact_decay():
+    # no absorption: mass of the zeroth moment conserved (periodic, no source)
+    par = so.starmap_init(P.gauss_periodic_2d(n=40, n_mom=5, std=0.1, tfinal=0.5))
+    m0 = so.starmap_solver(dict(par, tfinal=0.0, t_plot=[]), verbose=False)[0]["U"].sum()
+    out, p2 = so.starmap_solver(par, verbose=False, return_state=True)
+    assert abs(out[0]["U"].sum() - m0) <= 1e-13 * abs(m0)
+    # homogeneous absorption: mass decays by (1 - sa dt/2 E)^2 per step
+    par = so.starmap_init(P.gauss_periodic_3d(n=12, n_mom=3, std=0.3, tfinal=0.4))
+    m0 = so.starmap_solver(dict(par, tfinal=0.0, t_plot=[]), verbose=False)[0]["U"].sum()
+    out, p2 = so.starmap_solver(par, verbose=False, return_state=True)
+    dt, steps = p2["_oracle"]["dt"], p2["_oracle"]["step_count"]
+    f = np.exp(-0.1 * dt / 2)
+    assert abs(out[0]["U"].sum() - m0 * f ** (2 * steps)) <= 1e-12 * abs(m0)
+
+
+@pytest.mark.parametrize("N", [3, 7])
+def test_pn_equals_spn_boxes(N):
+    res = {}
+    for cl in ("P", "SP"):
+        prob = P.boxes_2d(n=32, n_mom=N, closure=cl, tfinal=0.25)
+        res[cl] = so.starmap_solver(so.starmap_init(prob), verbose=False)[0]["U"]
+    assert np.max(np.abs(res["P"] - res["SP"])) < 1e-13 * np.max(np.abs(res["P"]))
+
+
+def test_exp_filter_closed_form():
+    f = P.exp_filter(4, 10.0)
+    par = {"n_mom": 39}
+    dt = 0.0035
+    for k in range(0, 40):
+        want = np.exp(-dt * 10.0 * (k / 39) ** 4)
+        assert abs(f(par, dt, k) - want) <= 4e-16
+
+
+def test_output_interpolation_between_steps():
+    """The callback at a t_plot inside a step receives (1-l) U(t-dt) + l U(t)
+    with l = (t_plot - t + dt)/dt."""
+    got = {}
+
+    def output(par, x, y, U, step):
+        got[step] = U.copy()
+
+    prob = P.gauss_periodic_2d(n=20, n_mom=3, std=0.2, tfinal=0.2)
+    par = so.starmap_init(prob)
+    _, p2 = so.starmap_solver(dict(par, t_plot=[]), verbose=False, return_state=True)
+    dt = p2["_oracle"]["dt"]
+    tp = 2.4 * dt
+    outs = []
+    for nsteps in (2, 3):
+        outs.append(so.starmap_solver(dict(par, tfinal=(nsteps - 0.5) * dt, t_plot=[]),
+                                      verbose=False)[0]["U"])
+    so.starmap_solver(dict(par, t_plot=[tp], tfinal=tp, output=output, mom_output=1),
+                      verbose=False)
+    lam = (tp - 3 * dt + dt) / dt
+    want = (1 - lam) * outs[0] + lam * outs[1]
+    assert np.max(np.abs(got[1] - want)) < 1e-13 * np.max(np.abs(want))
