@@ -1,0 +1,158 @@
+/*
+ * starmap_hip.h -- C ABI of the MI355X (gfx950) StaRMAP time-stepping core.
+ *
+ * The reference runs the whole solver inside one MATLAB function,
+ * `output = starmap_solver(par)` (starmap_solver.m:1).  This library replaces
+ * its time loop (starmap_solver.m:159-475); everything above the loop -- the
+ * problem struct of starmap_init.m, the closures starmap_closure_pn.m /
+ * starmap_closure_spn.m, the evaluation of the user's function handles on the
+ * staggered grids -- stays with the caller (a MEX gateway, the Python mirror in
+ * starmap_amd/solver.py, or C/C++), which hands plain arrays down.
+ *
+ * Conventions
+ *   - Every call returns an SMP_* status; on failure smp_last_error() returns
+ *     a message (thread-local).  No C++ exception crosses this boundary.
+ *   - Host buffers belong to the caller and are only read/written during the
+ *     call.  The handle owns all device memory.
+ *   - A handle is not thread-safe.  Separate handles may run concurrently.
+ *   - Component indices j are 0-based (the reference's j-1).
+ *   - Field arrays use the reference's layout of U{j}: column-major over the
+ *     solver axes (x fastest) with the component's own extents, i.e. n(d)
+ *     cells on a cell-centred axis, n(d) faces (periodic) or n(d)+1 faces
+ *     (extrapolation) on a staggered axis (starmap_solver.m:113-126).
+ */
+#ifndef STARMAP_HIP_H
+#define STARMAP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define SMP_OK          0
+#define SMP_E_INVALID   1   /* bad argument or descriptor                       */
+#define SMP_E_HIP       2   /* a HIP runtime call failed                         */
+#define SMP_E_OOM       3   /* device or host allocation failed                  */
+#define SMP_E_STATE     4   /* call not valid in the handle's current state      */
+#define SMP_E_UNSTABLE  5   /* non-finite values detected in the state           */
+
+/* state precision */
+#define SMP_FP64 0
+#define SMP_FP32 1
+
+/* filter position, par.f_position (starmap_solver.m:62-66, :355, :417) */
+#define SMP_FILTER_NONE    0
+#define SMP_FILTER_SUBSTEP 1
+#define SMP_FILTER_FULL    2
+
+/* coefficient kinds for smp_set_coeff */
+#define SMP_COEF_SIGMA_A1 0   /* sA1: decay rate of the zeroth moment, grid 111  */
+#define SMP_COEF_SIGMA_T  1   /* sT1{gt,order}: decay rate of the other moments  */
+#define SMP_COEF_SOURCE   2   /* Q{j}                                             */
+#define SMP_COEF_FILTER   3   /* filterTerm{gt,f_order,slot}                      */
+
+typedef struct smp_problem smp_problem;
+
+typedef struct smp_desc {
+    int32_t n_sys;            /* number of moment components, length(par.Mz)    */
+    int32_t n[3];             /* cells per solver axis, par.n after lifting     */
+    int32_t bc[3];            /* 0 periodic, 1 extrapolation, par.bc            */
+    double  h[3];             /* grid spacing, (ax_hi-ax_lo)./n                 */
+    /* coupling matrices Mx, My, Mz in CSR over rows (targets); row_ptr has
+     * n_sys+1 entries; columns ascending within a row.  A direction with no
+     * coupling may pass NULL pointers.  Non-symmetric matrices are allowed. */
+    const int32_t *row_ptr[3];
+    const int32_t *col[3];
+    const double  *val[3];
+    const int32_t *mom_order; /* order used to look up sT (par.mom_order after
+                                 the isotropic override of :68-70)              */
+    const int32_t *f_order;   /* filter order per component, or NULL           */
+    int32_t f_position;       /* SMP_FILTER_*                                   */
+    int32_t precision;        /* SMP_FP64 or SMP_FP32                           */
+    int32_t device;           /* HIP device ordinal                             */
+} smp_desc;
+
+/* Last error message of the calling thread ("" if none). */
+const char *smp_last_error(void);
+
+/* Library version string. */
+const char *smp_version(void);
+
+/* Create a solver handle: validates the descriptor, computes the grid types
+ * of every component (starmap_solver.m:76-98), builds the substep schedules,
+ * allocates the state (zeroed) and uploads the coupling tables once.
+ * Replaces the setup part of starmap_solver.m:58-157. */
+int smp_create(const smp_desc *desc, smp_problem **out);
+
+/* Release all device memory of the handle. */
+int smp_destroy(smp_problem *h);
+
+/* Grid type (1 = cell centres, 2 = faces) of each component per solver axis,
+ * gt[3*j+d] -- gtx/gty/gtz of starmap_solver.m:95-98. */
+int smp_grid_types(const smp_problem *h, int32_t *gt);
+
+/* Extents of component j per solver axis (the size of U{j}). */
+int smp_component_shape(const smp_problem *h, int32_t j, int64_t shape[3]);
+
+/* Copy component j in / out (count = product of its shape), FP64 on the host
+ * whatever the state precision.  Replaces U{j} = ... (:140-146) and the
+ * final read-out of U (:484-499). */
+int smp_set_state(smp_problem *h, int32_t j, const double *src, int64_t count);
+int smp_get_state(smp_problem *h, int32_t j, double *dst, int64_t count);
+
+/* Set a coefficient (starmap_solver.m:214-252, :278-305).
+ *   SIGMA_A1: gt, order, slot ignored
+ *   SIGMA_T : gt[3] and order select sT1{gt,order}
+ *   SOURCE  : order = component j; field NULL clears the source
+ *   FILTER  : gt[3], order = filter order, slot 1/2 ('substep') or 0 ('full')
+ * field == NULL means the value is the scalar; otherwise field holds count
+ * values on the grid gt (for SOURCE: on the grid of component j).
+ * The decay factors E = expm1div(-sigma*dt/2) (:267-274) are recomputed on
+ * the device by the next smp_step whenever a sigma or dt changed. */
+int smp_set_coeff(smp_problem *h, int32_t kind, const int32_t gt[3], int32_t order,
+                  int32_t slot, const double *field, double scalar, int64_t count);
+
+/* Advance n_steps full time steps of size dt: the three Strang substeps
+ * [1 2 1] and the optional full filter (starmap_solver.m:307-421).  Kernels
+ * are queued on the handle's stream; the call returns after they finish. */
+int smp_step(smp_problem *h, double dt, int64_t n_steps);
+
+/* Output accumulation (starmap_solver.m:308, :425-428):
+ * smp_snapshot keeps U0 = U(comps) on the device; smp_interp writes
+ * (1-lambda)*U0 + lambda*U(comps) to the host, components concatenated in the
+ * reference layout (count = total elements). */
+int smp_snapshot(smp_problem *h, const int32_t *comps, int32_t n_comps);
+int smp_interp(smp_problem *h, double lambda, double *out, int64_t count);
+
+/* FP64 reductions over the current state, with wavefront-shuffle partial sums
+ * per workgroup: sums[j] = sum of U{j}; sq[j] = sum over cells of w*U{j}^2
+ * with the staggered trapezoid weights of starmap_2d_ex_l2norm.m:93-99
+ * (times prod(h)).  Either pointer may be NULL.  Also reports whether any
+ * value is non-finite (*nonfinite = 1). */
+int smp_observables(smp_problem *h, double *sums, double *sq, int32_t *nonfinite);
+
+/* Timing of the last smp_step call: elapsed milliseconds of all its kernels
+ * measured with HIP events on the handle's stream, and the time of the
+ * substep kernels alone (sum over launches) with their launch count. */
+int smp_last_step_timing(const smp_problem *h, double *total_ms, double *substep_ms,
+                         int64_t *substep_launches);
+
+/* The HIP stream (hipStream_t) the handle launches on. */
+void *smp_stream(const smp_problem *h);
+
+/* Device bytes the handle allocated. */
+int64_t smp_device_bytes(const smp_problem *h);
+
+/* Schedule statistics for inspection and tests: for substep kind 0 (odd
+ * targets) and 1 (even targets): number of groups, LDS slots, source tile
+ * loads per tile and distinct sources. */
+int smp_plan_stats(const smp_problem *h, int32_t kind, int32_t *n_groups, int32_t *n_slots,
+                   int32_t *n_loads, int32_t *n_sources);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* STARMAP_HIP_H */

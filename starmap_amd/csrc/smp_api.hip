@@ -1,0 +1,773 @@
+// C ABI of the StaRMAP time-stepping core (include/starmap_hip.h).
+//
+// Device layout (moment-major SoA): component j occupies one slab of
+// `slab` elements; inside a slab, cell (i0,i1,i2) over the kernel axes sits
+// at i0 + pitch0*(i1 + E1*i2), with E_e = n_e + bc_e so that the n+1 faces
+// of an extrapolation axis fit, and pitch0 padded to 16 elements.  Kernel
+// axes are the solver axes that carry cells or couplings (2D: solver y,z).
+// On a periodic axis, face f of the reference (at ax_lo+(f+1)h) is stored at
+// index (f+1) mod n, so that on every axis face s sits at ax_lo + s*h and
+// the difference to cell centres is U(s)-U(s-1) / F(c+1)-F(c).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/starmap_hip.h"
+#include "smp_kernels.h"
+#include "smp_plan.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(e_ == hipErrorOutOfMemory ? SMP_E_OOM : SMP_E_HIP,                \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));               \
+    } while (0)
+
+double expm1div_host(double x) {
+    // starmap_solver.m:531-535
+    if (std::fabs(x) > 2e-4) return (std::exp(x) - 1) / x;
+    return 1 + x * .5 + (x * x) / 6;
+}
+
+struct Field {
+    void *dev = nullptr;     // device field (slab layout) or null
+    double scalar = 0.0;
+    bool set = false;
+};
+
+using Key = std::tuple<int, int, int, int, int>;   // gt x,y,z, order, slot
+
+}  // namespace
+
+struct smp_problem {
+    smp::Geometry geo;
+    smp::TileShape ts;
+    smp::Plan plan[2];
+    int precision = SMP_FP64;
+    int elem = 8;
+    int f_position = SMP_FILTER_NONE;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int E[3] = {1, 1, 1};          // storage extent per kernel axis
+    int pitch0 = 1;
+    int64_t slab = 0;
+    int ntile[3] = {1, 1, 1};
+    int n_tiles = 0;
+    void *U = nullptr;
+    void *plan_dev[2] = {nullptr, nullptr};
+    size_t lds_bytes[2] = {0, 0};
+    std::vector<int32_t> mom_order, f_order;
+    Field sA1, EA;
+    std::map<Key, Field> sT, ET, F;
+    std::vector<Field> Q;
+    double dt_decay = std::nan("");
+    bool decay_dirty = true;
+    bool coef_dirty = true;
+    void *coefs = nullptr;
+    std::vector<int32_t> snap_comps;
+    void *snap = nullptr;
+    void *interp_buf = nullptr;
+    int64_t interp_cap = 0;
+    void *red_buf = nullptr;
+    std::vector<hipEvent_t> events;
+    double last_total_ms = 0, last_substep_ms = 0;
+    int64_t last_launches = 0;
+    int64_t dev_bytes = 0;
+    std::vector<void *> allocs;
+
+    ~smp_problem() {
+        for (void *p : allocs) (void)hipFree(p);
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    int dalloc(void **p, size_t bytes) {
+        hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+        if (e != hipSuccess) {
+            *p = nullptr;
+            return fail(e == hipErrorOutOfMemory ? SMP_E_OOM : SMP_E_HIP,
+                        std::string("hipMalloc(") + std::to_string(bytes) + "): " +
+                            hipGetErrorString(e));
+        }
+        allocs.push_back(*p);
+        dev_bytes += (int64_t)bytes;
+        return SMP_OK;
+    }
+
+    void dfree(void *p) {
+        if (!p) return;
+        auto it = std::find(allocs.begin(), allocs.end(), p);
+        if (it != allocs.end()) allocs.erase(it);
+        (void)hipFree(p);
+    }
+
+    int64_t numel(int j) const {
+        int64_t m = 1;
+        for (int d = 0; d < 3; ++d) m *= geo.ext_of(j, d);
+        return m;
+    }
+
+    int64_t numel_gt(const int gt[3]) const {
+        int64_t m = 1;
+        for (int d = 0; d < 3; ++d) m *= gt[d] == 1 ? geo.n[d] : geo.n[d] + geo.bc[d];
+        return m;
+    }
+
+    // Convert between the reference layout of a field on grid gt (FP64,
+    // column-major over solver axes) and the slab layout of type T.
+    template <typename T>
+    void pack(const int gt[3], const double *src, T *dst) const {
+        int e[3];
+        for (int d = 0; d < 3; ++d) e[d] = gt[d] == 1 ? geo.n[d] : geo.n[d] + geo.bc[d];
+        // per solver axis: map index -> (kernel axis stride contribution)
+        std::vector<int64_t> off[3];
+        for (int d = 0; d < 3; ++d) {
+            off[d].resize(e[d]);
+            int ka = -1;
+            for (int k = 0; k < geo.ndim; ++k)
+                if (geo.axis[k] == d) ka = k;
+            int64_t stride = ka == 0 ? 1 : ka == 1 ? pitch0 : (int64_t)pitch0 * E[1];
+            for (int i = 0; i < e[d]; ++i) {
+                int s = i;
+                if (geo.bc[d] == 0 && gt[d] == 2) s = (i + 1) % geo.n[d];
+                off[d][i] = ka < 0 ? 0 : s * stride;
+            }
+        }
+        std::fill(dst, dst + slab, (T)0);
+        int64_t p = 0;
+        for (int i3 = 0; i3 < e[2]; ++i3)
+            for (int i2 = 0; i2 < e[1]; ++i2) {
+                const int64_t b = off[2][i3] + off[1][i2];
+                for (int i1 = 0; i1 < e[0]; ++i1) dst[b + off[0][i1]] = (T)src[p++];
+            }
+    }
+
+    template <typename T>
+    void unpack(const int gt[3], const T *src, double *dst) const {
+        int e[3];
+        for (int d = 0; d < 3; ++d) e[d] = gt[d] == 1 ? geo.n[d] : geo.n[d] + geo.bc[d];
+        std::vector<int64_t> off[3];
+        for (int d = 0; d < 3; ++d) {
+            off[d].resize(e[d]);
+            int ka = -1;
+            for (int k = 0; k < geo.ndim; ++k)
+                if (geo.axis[k] == d) ka = k;
+            int64_t stride = ka == 0 ? 1 : ka == 1 ? pitch0 : (int64_t)pitch0 * E[1];
+            for (int i = 0; i < e[d]; ++i) {
+                int s = i;
+                if (geo.bc[d] == 0 && gt[d] == 2) s = (i + 1) % geo.n[d];
+                off[d][i] = ka < 0 ? 0 : s * stride;
+            }
+        }
+        int64_t p = 0;
+        for (int i3 = 0; i3 < e[2]; ++i3)
+            for (int i2 = 0; i2 < e[1]; ++i2) {
+                const int64_t b = off[2][i3] + off[1][i2];
+                for (int i1 = 0; i1 < e[0]; ++i1) dst[p++] = (double)src[b + off[0][i1]];
+            }
+    }
+
+    // upload a host FP64 field on grid gt into a (new or existing) slab
+    int upload_field(const int gt[3], const double *src, void **dev) {
+        if (!*dev) {
+            int rc = dalloc(dev, (size_t)slab * elem);
+            if (rc) return rc;
+        }
+        if (precision == SMP_FP64) {
+            std::vector<double> tmp(slab);
+            pack<double>(gt, src, tmp.data());
+            HIP_TRY(hipMemcpy(*dev, tmp.data(), (size_t)slab * 8, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> tmp(slab);
+            pack<float>(gt, src, tmp.data());
+            HIP_TRY(hipMemcpy(*dev, tmp.data(), (size_t)slab * 4, hipMemcpyHostToDevice));
+        }
+        return SMP_OK;
+    }
+};
+
+extern "C" {
+
+const char *smp_last_error(void) { return g_err.c_str(); }
+
+const char *smp_version(void) { return "starmap-hip 0.1 (gfx950)"; }
+
+int smp_create(const smp_desc *d, smp_problem **out) {
+    g_err.clear();
+    if (!d || !out) return fail(SMP_E_INVALID, "null argument");
+    *out = nullptr;
+    if (d->precision != SMP_FP64 && d->precision != SMP_FP32)
+        return fail(SMP_E_INVALID, "precision must be SMP_FP64 or SMP_FP32");
+    if (d->f_position < SMP_FILTER_NONE || d->f_position > SMP_FILTER_FULL)
+        return fail(SMP_E_INVALID, "invalid f_position");
+    if (d->f_position != SMP_FILTER_NONE && !d->f_order)
+        return fail(SMP_E_INVALID, "a filter needs f_order");
+    if (!d->mom_order) return fail(SMP_E_INVALID, "mom_order is required");
+    std::unique_ptr<smp_problem> h(new smp_problem());
+    h->geo.n_sys = d->n_sys;
+    for (int a = 0; a < 3; ++a) {
+        h->geo.n[a] = d->n[a];
+        h->geo.bc[a] = d->bc[a];
+        h->geo.h[a] = d->h[a];
+        if (d->row_ptr[a] && d->n_sys > 0) {
+            int nnz = d->row_ptr[a][d->n_sys];
+            if (nnz > 0) {
+                if (!d->col[a] || !d->val[a]) return fail(SMP_E_INVALID, "CSR col/val missing");
+                h->geo.M[a].rp.assign(d->row_ptr[a], d->row_ptr[a] + d->n_sys + 1);
+                h->geo.M[a].ci.assign(d->col[a], d->col[a] + nnz);
+                h->geo.M[a].cv.assign(d->val[a], d->val[a] + nnz);
+            }
+        }
+    }
+    std::string err = smp::build_geometry(h->geo);
+    if (!err.empty()) return fail(SMP_E_INVALID, err);
+    h->precision = d->precision;
+    h->elem = d->precision == SMP_FP64 ? 8 : 4;
+    h->f_position = d->f_position;
+    h->device = d->device;
+    h->mom_order.assign(d->mom_order, d->mom_order + d->n_sys);
+    if (d->f_order) h->f_order.assign(d->f_order, d->f_order + d->n_sys);
+    h->Q.resize(d->n_sys);
+
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (d->device < 0 || d->device >= ndev) return fail(SMP_E_INVALID, "no such HIP device");
+    HIP_TRY(hipSetDevice(d->device));
+    HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+
+    // storage geometry
+    for (int e = 0; e < 3; ++e) {
+        h->E[e] = 1;
+        if (e < h->geo.ndim) {
+            int ax = h->geo.axis[e];
+            int m = 1;
+            for (int j = 0; j < d->n_sys; ++j) m = std::max(m, h->geo.ext_of(j, ax));
+            h->E[e] = m;
+        }
+    }
+    h->pitch0 = h->geo.ndim > 1 ? ((h->E[0] + 15) / 16) * 16 : h->E[0];
+    h->slab = (int64_t)h->pitch0 * h->E[1] * h->E[2];
+    h->slab = ((h->slab + 63) / 64) * 64;
+    h->ts = smp::choose_tile(h->geo, h->elem);
+    h->n_tiles = 1;
+    for (int e = 0; e < 3; ++e) {
+        h->ntile[e] = (h->E[e] + h->ts.tile[e] - 1) / h->ts.tile[e];
+        h->n_tiles *= h->ntile[e];
+    }
+    // schedules
+    const int lds_budget = 160 * 1024;
+    const int cap = lds_budget / (h->ts.slot_elems * h->elem);
+    for (int kind = 0; kind < 2; ++kind) {
+        err = smp::build_plan(h->geo, h->ts, kind, cap, 4, h->plan[kind]);
+        if (!err.empty()) return fail(SMP_E_INVALID, err);
+        const smp::Plan &p = h->plan[kind];
+        h->lds_bytes[kind] = (size_t)std::max(p.n_slots, 1) * h->ts.slot_elems * h->elem;
+        // device copy: groups | loads | targets | terms, each 16-byte aligned
+        size_t s0 = p.groups.size() * sizeof(smp::DevGroup);
+        size_t s1 = p.loads.size() * sizeof(smp::DevLoad);
+        size_t s2 = p.targets.size() * sizeof(smp::DevTarget);
+        size_t s3 = p.terms.size() * sizeof(smp::DevTerm);
+        auto al = [](size_t v) { return (v + 15) / 16 * 16; };
+        size_t bytes = al(s0) + al(s1) + al(s2) + al(s3) + 16;
+        int rc = h->dalloc(&h->plan_dev[kind], bytes);
+        if (rc) return rc;
+        std::vector<unsigned char> buf(bytes, 0);
+        size_t o = 0;
+        std::memcpy(buf.data() + o, p.groups.data(), s0);
+        o += al(s0);
+        std::memcpy(buf.data() + o, p.loads.data(), s1);
+        o += al(s1);
+        std::memcpy(buf.data() + o, p.targets.data(), s2);
+        o += al(s2);
+        std::memcpy(buf.data() + o, p.terms.data(), s3);
+        HIP_TRY(hipMemcpy(h->plan_dev[kind], buf.data(), bytes, hipMemcpyHostToDevice));
+    }
+    // state
+    int rc = h->dalloc(&h->U, (size_t)h->slab * h->elem * d->n_sys);
+    if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(h->U, 0, (size_t)h->slab * h->elem * d->n_sys, h->stream));
+    rc = h->dalloc(&h->coefs, sizeof(smp::CoefRec<double>) * d->n_sys);
+    if (rc) return rc;
+    rc = h->dalloc(&h->red_buf, sizeof(double) * (3 * 1024 + 3));
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    *out = h.release();
+    return SMP_OK;
+}
+
+int smp_destroy(smp_problem *h) {
+    if (!h) return SMP_OK;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    delete h;
+    return SMP_OK;
+}
+
+int smp_grid_types(const smp_problem *h, int32_t *gt) {
+    if (!h || !gt) return fail(SMP_E_INVALID, "null argument");
+    std::memcpy(gt, h->geo.gt.data(), sizeof(int32_t) * 3 * h->geo.n_sys);
+    return SMP_OK;
+}
+
+int smp_component_shape(const smp_problem *h, int32_t j, int64_t shape[3]) {
+    if (!h || !shape) return fail(SMP_E_INVALID, "null argument");
+    if (j < 0 || j >= h->geo.n_sys) return fail(SMP_E_INVALID, "component out of range");
+    for (int d = 0; d < 3; ++d) shape[d] = h->geo.ext_of(j, d);
+    return SMP_OK;
+}
+
+int smp_set_state(smp_problem *h, int32_t j, const double *src, int64_t count) {
+    if (!h || !src) return fail(SMP_E_INVALID, "null argument");
+    if (j < 0 || j >= h->geo.n_sys) return fail(SMP_E_INVALID, "component out of range");
+    if (count != h->numel(j)) return fail(SMP_E_INVALID, "count does not match the component shape");
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    const int gt[3] = {h->geo.gt[3 * j], h->geo.gt[3 * j + 1], h->geo.gt[3 * j + 2]};
+    char *dst = (char *)h->U + (size_t)j * h->slab * h->elem;
+    if (h->precision == SMP_FP64) {
+        std::vector<double> tmp(h->slab);
+        h->pack<double>(gt, src, tmp.data());
+        HIP_TRY(hipMemcpy(dst, tmp.data(), (size_t)h->slab * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> tmp(h->slab);
+        h->pack<float>(gt, src, tmp.data());
+        HIP_TRY(hipMemcpy(dst, tmp.data(), (size_t)h->slab * 4, hipMemcpyHostToDevice));
+    }
+    return SMP_OK;
+}
+
+int smp_get_state(smp_problem *h, int32_t j, double *dst, int64_t count) {
+    if (!h || !dst) return fail(SMP_E_INVALID, "null argument");
+    if (j < 0 || j >= h->geo.n_sys) return fail(SMP_E_INVALID, "component out of range");
+    if (count != h->numel(j)) return fail(SMP_E_INVALID, "count does not match the component shape");
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    const int gt[3] = {h->geo.gt[3 * j], h->geo.gt[3 * j + 1], h->geo.gt[3 * j + 2]};
+    const char *src = (const char *)h->U + (size_t)j * h->slab * h->elem;
+    if (h->precision == SMP_FP64) {
+        std::vector<double> tmp(h->slab);
+        HIP_TRY(hipMemcpy(tmp.data(), src, (size_t)h->slab * 8, hipMemcpyDeviceToHost));
+        h->unpack<double>(gt, tmp.data(), dst);
+    } else {
+        std::vector<float> tmp(h->slab);
+        HIP_TRY(hipMemcpy(tmp.data(), src, (size_t)h->slab * 4, hipMemcpyDeviceToHost));
+        h->unpack<float>(gt, tmp.data(), dst);
+    }
+    return SMP_OK;
+}
+
+int smp_set_coeff(smp_problem *h, int32_t kind, const int32_t gt_in[3], int32_t order,
+                  int32_t slot, const double *field, double scalar, int64_t count) {
+    if (!h) return fail(SMP_E_INVALID, "null handle");
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    int gt[3] = {1, 1, 1};
+    if (kind == SMP_COEF_SIGMA_T || kind == SMP_COEF_FILTER) {
+        if (!gt_in) return fail(SMP_E_INVALID, "grid type required");
+        for (int d = 0; d < 3; ++d) {
+            if (gt_in[d] != 1 && gt_in[d] != 2) return fail(SMP_E_INVALID, "grid type must be 1 or 2");
+            gt[d] = gt_in[d];
+        }
+    }
+    Field *f = nullptr;
+    switch (kind) {
+        case SMP_COEF_SIGMA_A1:
+            f = &h->sA1;
+            h->decay_dirty = true;
+            break;
+        case SMP_COEF_SIGMA_T:
+            f = &h->sT[Key(gt[0], gt[1], gt[2], order, 0)];
+            h->decay_dirty = true;
+            break;
+        case SMP_COEF_SOURCE:
+            if (order < 0 || order >= h->geo.n_sys) return fail(SMP_E_INVALID, "component out of range");
+            f = &h->Q[order];
+            for (int d = 0; d < 3; ++d) gt[d] = h->geo.gt[3 * order + d];
+            if (!field) {
+                // Q = 0 (scalar sources are expanded to a field)
+                if (scalar == 0.0) {
+                    f->set = false;
+                    h->coef_dirty = true;
+                    return SMP_OK;
+                }
+            }
+            break;
+        case SMP_COEF_FILTER:
+            if (slot < 0 || slot > 2) return fail(SMP_E_INVALID, "filter slot must be 0, 1 or 2");
+            f = &h->F[Key(gt[0], gt[1], gt[2], order, slot)];
+            break;
+        default:
+            return fail(SMP_E_INVALID, "unknown coefficient kind");
+    }
+    const int64_t need = h->numel_gt(gt);
+    if (field && count != need) return fail(SMP_E_INVALID, "field count does not match its grid");
+    if (field) {
+        int rc = h->upload_field(gt, field, &f->dev);
+        if (rc) return rc;
+    } else if (kind == SMP_COEF_SOURCE) {
+        std::vector<double> tmp(need, scalar);
+        int rc = h->upload_field(gt, tmp.data(), &f->dev);
+        if (rc) return rc;
+    } else {
+        if (f->dev) {
+            h->dfree(f->dev);
+            f->dev = nullptr;
+        }
+        f->scalar = scalar;
+    }
+    f->set = true;
+    h->coef_dirty = true;
+    return SMP_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+int refresh_decay(smp_problem *h, double dt) {
+    auto one = [&](const Field &sig, Field &E, const int gt[3]) -> int {
+        (void)gt;
+        if (sig.dev) {
+            if (!E.dev) {
+                int rc = h->dalloc(&E.dev, (size_t)h->slab * sizeof(T));
+                if (rc) return rc;
+            }
+            HIP_TRY(smp::launch_decay<T>((const T *)sig.dev, (T *)E.dev, h->slab, dt, h->stream));
+        } else {
+            if (E.dev) {
+                h->dfree(E.dev);
+                E.dev = nullptr;
+            }
+            E.scalar = expm1div_host(-sig.scalar * dt / 2);
+        }
+        E.set = true;
+        return SMP_OK;
+    };
+    if (!h->sA1.set) return fail(SMP_E_STATE, "sigma_a1 not set");
+    const int g111[3] = {1, 1, 1};
+    int rc = one(h->sA1, h->EA, g111);
+    if (rc) return rc;
+    for (auto &kv : h->sT) {
+        const int gt[3] = {std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first)};
+        rc = one(kv.second, h->ET[kv.first], gt);
+        if (rc) return rc;
+    }
+    h->dt_decay = dt;
+    h->decay_dirty = false;
+    h->coef_dirty = true;
+    return SMP_OK;
+}
+
+template <typename T>
+int refresh_coefs(smp_problem *h) {
+    const int n = h->geo.n_sys;
+    std::vector<smp::CoefRec<T>> rec(n);
+    for (int j = 0; j < n; ++j) {
+        smp::CoefRec<T> &c = rec[j];
+        std::memset(&c, 0, sizeof(c));
+        const Field *sig, *E;
+        if (j == 0) {
+            sig = &h->sA1;
+            E = &h->EA;
+        } else {
+            Key k(h->geo.gt[3 * j], h->geo.gt[3 * j + 1], h->geo.gt[3 * j + 2], h->mom_order[j], 0);
+            auto it = h->sT.find(k);
+            if (it == h->sT.end() || !it->second.set) {
+                return fail(SMP_E_STATE, "sigma_t not set for the grid type and order of component " +
+                                             std::to_string(j));
+            }
+            sig = &it->second;
+            E = &h->ET[k];
+        }
+        c.sig = (const T *)sig->dev;
+        c.sig_s = sig->scalar;
+        c.E = (const T *)E->dev;
+        c.E_s = E->scalar;
+        c.Q = h->Q[j].set ? (const T *)h->Q[j].dev : nullptr;
+        c.F_s[0] = c.F_s[1] = 1.0;
+        c.filt = 0;
+        if (j > 0 && h->f_position == SMP_FILTER_SUBSTEP) {
+            c.filt = 1;
+            for (int s = 0; s < 2; ++s) {
+                Key k(h->geo.gt[3 * j], h->geo.gt[3 * j + 1], h->geo.gt[3 * j + 2], h->f_order[j], s + 1);
+                auto it = h->F.find(k);
+                if (it == h->F.end() || !it->second.set)
+                    return fail(SMP_E_STATE, "substep filter not set for component " + std::to_string(j));
+                c.F[s] = (const T *)it->second.dev;
+                c.F_s[s] = it->second.scalar;
+            }
+        } else if (j > 0 && h->f_position == SMP_FILTER_FULL) {
+            Key k(h->geo.gt[3 * j], h->geo.gt[3 * j + 1], h->geo.gt[3 * j + 2], h->f_order[j], 0);
+            auto it = h->F.find(k);
+            if (it == h->F.end() || !it->second.set)
+                return fail(SMP_E_STATE, "full filter not set for component " + std::to_string(j));
+            c.F[0] = (const T *)it->second.dev;
+            c.F_s[0] = it->second.scalar;
+        }
+    }
+    HIP_TRY(hipMemcpy(h->coefs, rec.data(), sizeof(smp::CoefRec<T>) * n, hipMemcpyHostToDevice));
+    h->coef_dirty = false;
+    return SMP_OK;
+}
+
+template <typename T>
+smp::SubstepArgs<T> make_args(smp_problem *h, int kind, double dt, int fslot) {
+    smp::SubstepArgs<T> a;
+    std::memset(&a, 0, sizeof(a));
+    a.U = (T *)h->U;
+    a.slab = h->slab;
+    a.pitch0 = h->pitch0;
+    a.ext1 = h->E[1];
+    for (int e = 0; e < 3; ++e) {
+        if (e < h->geo.ndim) {
+            a.n[e] = h->geo.n[h->geo.axis[e]];
+            a.bc[e] = h->geo.bc[h->geo.axis[e]];
+        } else {
+            a.n[e] = 1;
+            a.bc[e] = 0;
+        }
+        a.tile[e] = h->ts.tile[e];
+        a.ntile[e] = h->ntile[e];
+        a.lds[e] = h->ts.lds[e];
+    }
+    a.slot_elems = h->ts.slot_elems;
+    a.n_tiles = h->n_tiles;
+    const smp::Plan &p = h->plan[kind];
+    auto al = [](size_t v) { return (v + 15) / 16 * 16; };
+    char *base = (char *)h->plan_dev[kind];
+    size_t s0 = p.groups.size() * sizeof(smp::DevGroup);
+    size_t s1 = p.loads.size() * sizeof(smp::DevLoad);
+    size_t s2 = p.targets.size() * sizeof(smp::DevTarget);
+    a.groups = (const smp::DevGroup *)base;
+    a.loads = (const smp::DevLoad *)(base + al(s0));
+    a.targets = (const smp::DevTarget *)(base + al(s0) + al(s1));
+    a.terms = (const smp::DevTerm *)(base + al(s0) + al(s1) + al(s2));
+    a.ngroups = (int)p.groups.size();
+    a.coefs = (const smp::CoefRec<T> *)h->coefs;
+    a.half_dt = dt / 2;
+    a.fslot = fslot;
+    return a;
+}
+
+template <typename T>
+int run_steps(smp_problem *h, double dt, int64_t n_steps) {
+    if (h->decay_dirty || !(h->dt_decay == dt)) {
+        int rc = refresh_decay<T>(h, dt);
+        if (rc) return rc;
+    }
+    if (h->coef_dirty) {
+        int rc = refresh_coefs<T>(h);
+        if (rc) return rc;
+    }
+    smp::SubstepArgs<T> odd1 = make_args<T>(h, 0, dt, 0);
+    smp::SubstepArgs<T> even = make_args<T>(h, 1, dt, 0);
+    smp::SubstepArgs<T> odd3 = make_args<T>(h, 0, dt, 1);
+    const int per_step = 3;
+    const int64_t launches = n_steps * per_step;
+    // a start/end event pair around every substep launch (up to a cap):
+    // their differences are the substep kernel durations on this stream
+    const int64_t timed = std::min<int64_t>(launches, 3 * 4096);
+    // two extra events bracket the whole call
+    while ((int64_t)h->events.size() < 2 * timed + 2) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        h->events.push_back(e);
+    }
+    hipEvent_t ev_start = h->events[0], ev_end = h->events[1];
+    HIP_TRY(hipEventRecord(ev_start, h->stream));
+    int64_t li = 0;
+    for (int64_t s = 0; s < n_steps; ++s) {
+        const smp::SubstepArgs<T> *seq[3] = {&odd1, &even, &odd3};
+        for (int k = 0; k < 3; ++k) {
+            const int kind = k == 1 ? 1 : 0;
+            if (li < timed) HIP_TRY(hipEventRecord(h->events[2 + 2 * li], h->stream));
+            HIP_TRY(smp::launch_substep<T>(*seq[k], h->ts.threads, h->lds_bytes[kind], h->stream));
+            if (li < timed) HIP_TRY(hipEventRecord(h->events[3 + 2 * li], h->stream));
+            ++li;
+        }
+        if (h->f_position == SMP_FILTER_FULL) {
+            for (int j = 1; j < h->geo.n_sys; ++j) {
+                Key k(h->geo.gt[3 * j], h->geo.gt[3 * j + 1], h->geo.gt[3 * j + 2], h->f_order[j], 0);
+                const Field &f = h->F[k];
+                HIP_TRY(smp::launch_scale<T>((T *)h->U + (size_t)j * h->slab, (const T *)f.dev, f.scalar,
+                                             h->slab, h->stream));
+            }
+        }
+    }
+    HIP_TRY(hipEventRecord(ev_end, h->stream));
+    HIP_TRY(hipEventSynchronize(ev_end));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, ev_start, ev_end));
+    h->last_total_ms = ms;
+    double sub_ms = 0;
+    for (int64_t i = 0; i < timed; ++i) {
+        float m = 0;
+        HIP_TRY(hipEventElapsedTime(&m, h->events[2 + 2 * i], h->events[3 + 2 * i]));
+        sub_ms += m;
+    }
+    h->last_substep_ms = sub_ms;
+    h->last_launches = timed;
+    return SMP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int smp_step(smp_problem *h, double dt, int64_t n_steps) {
+    if (!h) return fail(SMP_E_INVALID, "null handle");
+    if (!(dt > 0) || !std::isfinite(dt)) return fail(SMP_E_INVALID, "dt must be positive and finite");
+    if (n_steps < 0) return fail(SMP_E_INVALID, "n_steps must be >= 0");
+    HIP_TRY(hipSetDevice(h->device));
+    if (n_steps == 0) return SMP_OK;
+    return h->precision == SMP_FP64 ? run_steps<double>(h, dt, n_steps)
+                                    : run_steps<float>(h, dt, n_steps);
+}
+
+int smp_snapshot(smp_problem *h, const int32_t *comps, int32_t n_comps) {
+    if (!h || (n_comps > 0 && !comps)) return fail(SMP_E_INVALID, "null argument");
+    if (n_comps < 0) return fail(SMP_E_INVALID, "n_comps must be >= 0");
+    for (int i = 0; i < n_comps; ++i)
+        if (comps[i] < 0 || comps[i] >= h->geo.n_sys) return fail(SMP_E_INVALID, "component out of range");
+    HIP_TRY(hipSetDevice(h->device));
+    if ((int)h->snap_comps.size() != n_comps) {
+        h->dfree(h->snap);
+        h->snap = nullptr;
+        if (n_comps > 0) {
+            int rc = h->dalloc(&h->snap, (size_t)n_comps * h->slab * h->elem);
+            if (rc) return rc;
+        }
+    }
+    h->snap_comps.assign(comps, comps + n_comps);
+    for (int i = 0; i < n_comps; ++i)
+        HIP_TRY(hipMemcpyAsync((char *)h->snap + (size_t)i * h->slab * h->elem,
+                               (const char *)h->U + (size_t)comps[i] * h->slab * h->elem,
+                               (size_t)h->slab * h->elem, hipMemcpyDeviceToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return SMP_OK;
+}
+
+int smp_interp(smp_problem *h, double lambda, double *out, int64_t count) {
+    if (!h || !out) return fail(SMP_E_INVALID, "null argument");
+    if (h->snap_comps.empty()) return fail(SMP_E_STATE, "no snapshot taken");
+    int64_t total = 0;
+    for (int j : h->snap_comps) total += h->numel(j);
+    if (count != total) return fail(SMP_E_INVALID, "count does not match the snapshot components");
+    HIP_TRY(hipSetDevice(h->device));
+    if (h->interp_cap < h->slab) {
+        h->dfree(h->interp_buf);
+        h->interp_buf = nullptr;
+        int rc = h->dalloc(&h->interp_buf, (size_t)h->slab * sizeof(double));
+        if (rc) return rc;
+        h->interp_cap = h->slab;
+    }
+    std::vector<double> tmp(h->slab);
+    int64_t pos = 0;
+    for (size_t i = 0; i < h->snap_comps.size(); ++i) {
+        const int j = h->snap_comps[i];
+        if (h->precision == SMP_FP64) {
+            HIP_TRY(smp::launch_interp<double>((const double *)h->snap + i * h->slab,
+                                               (const double *)h->U + (size_t)j * h->slab,
+                                               (double *)h->interp_buf, h->slab, lambda, h->stream));
+        } else {
+            HIP_TRY(smp::launch_interp<float>((const float *)h->snap + i * h->slab,
+                                              (const float *)h->U + (size_t)j * h->slab,
+                                              (double *)h->interp_buf, h->slab, lambda, h->stream));
+        }
+        HIP_TRY(hipMemcpyAsync(tmp.data(), h->interp_buf, (size_t)h->slab * sizeof(double),
+                               hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        const int gt[3] = {h->geo.gt[3 * j], h->geo.gt[3 * j + 1], h->geo.gt[3 * j + 2]};
+        h->unpack<double>(gt, tmp.data(), out + pos);
+        pos += h->numel(j);
+    }
+    return SMP_OK;
+}
+
+int smp_observables(smp_problem *h, double *sums, double *sq, int32_t *nonfinite) {
+    if (!h) return fail(SMP_E_INVALID, "null handle");
+    HIP_TRY(hipSetDevice(h->device));
+    const int blocks = 1024;
+    double *part = (double *)h->red_buf;
+    double *res = part + 3 * blocks;
+    double cell = 1;
+    for (int e = 0; e < h->geo.ndim; ++e) cell *= h->geo.h[h->geo.axis[e]];
+    int bad = 0;
+    for (int j = 0; j < h->geo.n_sys; ++j) {
+        smp::ReduceGeom g;
+        std::memset(&g, 0, sizeof(g));
+        for (int e = 0; e < 3; ++e) {
+            if (e < h->geo.ndim) {
+                const int ax = h->geo.axis[e];
+                g.ext[e] = h->geo.ext_of(j, ax);
+                g.stag[e] = g.ext[e] != h->geo.n[ax];
+            } else {
+                g.ext[e] = 1;
+                g.stag[e] = 0;
+            }
+        }
+        g.pitch0 = h->pitch0;
+        g.ext1 = h->E[1];
+        g.cell = cell;
+        if (h->precision == SMP_FP64)
+            HIP_TRY(smp::launch_reduce<double>((const double *)h->U + (size_t)j * h->slab, g, part,
+                                               blocks, h->stream));
+        else
+            HIP_TRY(smp::launch_reduce<float>((const float *)h->U + (size_t)j * h->slab, g, part,
+                                              blocks, h->stream));
+        HIP_TRY(smp::launch_reduce_final(part, blocks, res, h->stream));
+        double r[3];
+        HIP_TRY(hipMemcpyAsync(r, res, sizeof(r), hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        if (sums) sums[j] = r[0];
+        if (sq) sq[j] = r[1];
+        if (r[2] != 0) bad = 1;
+    }
+    if (nonfinite) *nonfinite = bad;
+    return SMP_OK;
+}
+
+int smp_last_step_timing(const smp_problem *h, double *total_ms, double *substep_ms,
+                         int64_t *substep_launches) {
+    if (!h) return fail(SMP_E_INVALID, "null handle");
+    if (total_ms) *total_ms = h->last_total_ms;
+    if (substep_ms) *substep_ms = h->last_substep_ms;
+    if (substep_launches) *substep_launches = h->last_launches;
+    return SMP_OK;
+}
+
+void *smp_stream(const smp_problem *h) { return h ? (void *)h->stream : nullptr; }
+
+int64_t smp_device_bytes(const smp_problem *h) { return h ? h->dev_bytes : 0; }
+
+int smp_plan_stats(const smp_problem *h, int32_t kind, int32_t *n_groups, int32_t *n_slots,
+                   int32_t *n_loads, int32_t *n_sources) {
+    if (!h) return fail(SMP_E_INVALID, "null handle");
+    if (kind != 0 && kind != 1) return fail(SMP_E_INVALID, "kind must be 0 or 1");
+    const smp::Plan &p = h->plan[kind];
+    if (n_groups) *n_groups = (int32_t)p.groups.size();
+    if (n_slots) *n_slots = p.n_slots;
+    if (n_loads) *n_loads = (int32_t)p.loads.size();
+    if (n_sources) *n_sources = p.n_sources;
+    return SMP_OK;
+}
+
+}  // extern "C"

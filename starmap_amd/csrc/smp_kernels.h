@@ -1,0 +1,73 @@
+// Device kernels of the StaRMAP time-stepping core and their launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "smp_plan.h"
+
+namespace smp {
+
+// Per-component coefficients of one half-step update.  A null field pointer
+// means the matching scalar applies everywhere.
+template <typename T>
+struct CoefRec {
+    const T *sig;     // decay rate: sA1 (j=0) or sT1{gt,order}
+    const T *E;       // expm1div(-sig*dt/2)
+    const T *Q;       // source, or null for Q = 0
+    const T *F[2];    // substep filter slots k=1,2 (or full filter in F[0])
+    double sig_s, E_s;
+    double F_s[2];
+    int32_t filt;     // apply substep filter after each application
+    int32_t pad;
+};
+
+template <typename T>
+struct SubstepArgs {
+    T *U;
+    int64_t slab;             // elements per component
+    int32_t pitch0, ext1;     // storage index = i0 + pitch0*(i1 + ext1*i2)
+    int32_t n[3], bc[3];      // per kernel axis
+    int32_t tile[3], ntile[3], lds[3];
+    int32_t slot_elems;
+    int32_t n_tiles;          // total tiles
+    int32_t ngroups;
+    const DevGroup *groups;
+    const DevLoad *loads;
+    const DevTarget *targets;
+    const DevTerm *terms;
+    const CoefRec<T> *coefs;
+    double half_dt;
+    int32_t fslot;            // filter slot of single-application targets
+};
+
+template <typename T>
+hipError_t launch_substep(const SubstepArgs<T> &a, int threads, size_t lds_bytes,
+                          hipStream_t s);
+
+template <typename T>
+hipError_t launch_decay(const T *sig, T *E, int64_t count, double dt, hipStream_t s);
+
+template <typename T>
+hipError_t launch_scale(T *U, const T *F, double F_s, int64_t count, hipStream_t s);
+
+template <typename T>
+hipError_t launch_interp(const T *U0, const T *U, double *out, int64_t count, double lambda,
+                         hipStream_t s);
+
+// FP64 partial sums of one component: per block (sum, weighted sum of
+// squares, non-finite count) into part[3*block]; weights from the staggered
+// trapezoid rule per kernel axis.
+struct ReduceGeom {
+    int32_t ext[3];           // component extent per kernel axis
+    int32_t pitch0, ext1;
+    int32_t stag[3];          // 1 if the component has n+1 points on this axis
+    double cell;              // prod(h)
+};
+
+template <typename T>
+hipError_t launch_reduce(const T *U, const ReduceGeom &g, double *part, int blocks,
+                         hipStream_t s);
+
+hipError_t launch_reduce_final(const double *part, int blocks, double *out3, hipStream_t s);
+
+}  // namespace smp

@@ -1,0 +1,236 @@
+// Host-side planning for the StaRMAP substep kernels.  See smp_plan.h.
+#include "smp_plan.h"
+
+#include <algorithm>
+#include <deque>
+#include <limits>
+#include <set>
+#include <sstream>
+
+namespace smp {
+
+std::string build_geometry(Geometry &g) {
+    if (g.n_sys <= 0) return "n_sys must be positive";
+    for (int d = 0; d < 3; ++d) {
+        if (g.n[d] < 1) return "every n(d) must be >= 1";
+        if (g.bc[d] != 0 && g.bc[d] != 1) return "bc(d) must be 0 (periodic) or 1 (extrapolation)";
+        if (!(g.h[d] > 0)) return "every h(d) must be positive";
+        const Csr &M = g.M[d];
+        if (M.rp.empty()) continue;
+        if ((int)M.rp.size() != g.n_sys + 1) return "row_ptr must have n_sys+1 entries";
+        if (M.rp[0] != 0 || M.rp[g.n_sys] != (int)M.ci.size()) return "row_ptr inconsistent with col";
+        for (int j = 0; j < g.n_sys; ++j) {
+            if (M.rp[j + 1] < M.rp[j]) return "row_ptr not non-decreasing";
+            for (int p = M.rp[j]; p < M.rp[j + 1]; ++p) {
+                if (M.ci[p] < 0 || M.ci[p] >= g.n_sys) return "column index out of range";
+                if (p > M.rp[j] && M.ci[p] <= M.ci[p - 1]) return "columns must ascend within a row";
+            }
+        }
+    }
+    // Grid types: component 1 lives on the (1,1,1) grid; a coupling along
+    // axis d connects components whose grid types differ exactly in d.  The
+    // reference reaches the same assignment by its fixed-point iteration
+    // over the eight classes (starmap_solver.m:80-98).
+    g.gt.assign(3 * g.n_sys, 0);
+    std::vector<char> seen(g.n_sys, 0);
+    std::deque<int> queue;
+    g.gt[0] = g.gt[1] = g.gt[2] = 1;
+    seen[0] = 1;
+    queue.push_back(0);
+    while (!queue.empty()) {
+        int j = queue.front();
+        queue.pop_front();
+        for (int d = 0; d < 3; ++d) {
+            const Csr &M = g.M[d];
+            if (M.rp.empty()) continue;
+            for (int p = M.rp[j]; p < M.rp[j + 1]; ++p) {
+                int k = M.ci[p];
+                int want[3] = {g.gt[3 * j], g.gt[3 * j + 1], g.gt[3 * j + 2]};
+                want[d] = 3 - want[d];
+                if (seen[k]) {
+                    for (int e = 0; e < 3; ++e)
+                        if (g.gt[3 * k + e] != want[e]) {
+                            std::ostringstream os;
+                            os << "coupling is not bipartite: components " << j << " and " << k
+                               << " along axis " << d;
+                            return os.str();
+                        }
+                } else {
+                    for (int e = 0; e < 3; ++e) g.gt[3 * k + e] = want[e];
+                    seen[k] = 1;
+                    queue.push_back(k);
+                }
+            }
+        }
+    }
+    for (int j = 0; j < g.n_sys; ++j)
+        if (!seen[j]) {
+            std::ostringstream os;
+            os << "component " << j << " is not coupled to component 0";
+            return os.str();
+        }
+    // kernel axes: solver axes that carry a coupling or more than one cell
+    g.ndim = 0;
+    for (int d = 0; d < 3; ++d)
+        if (!g.M[d].empty() || g.n[d] > 1) g.axis[g.ndim++] = d;
+    if (g.ndim == 0) g.axis[g.ndim++] = 2;
+    for (int e = g.ndim; e < 3; ++e) g.axis[e] = -1;
+    g.even.clear();
+    g.odd.clear();
+    for (int j = 0; j < g.n_sys; ++j) {
+        int par = (g.gt[3 * j] - 1) + (g.gt[3 * j + 1] - 1) + (g.gt[3 * j + 2] - 1);
+        (par % 2 ? g.odd : g.even).push_back(j);
+    }
+    return "";
+}
+
+TileShape choose_tile(const Geometry &g, int elem_bytes) {
+    (void)elem_bytes;
+    TileShape ts;
+    if (g.ndim == 1) {
+        ts.tile[0] = 256;
+    } else if (g.ndim == 2) {
+        ts.tile[0] = 64;
+        ts.tile[1] = 4;
+    } else {
+        ts.tile[0] = 32;
+        ts.tile[1] = 4;
+        ts.tile[2] = 2;
+    }
+    ts.threads = ts.tile[0] * ts.tile[1] * ts.tile[2];
+    for (int e = 0; e < 3; ++e) ts.lds[e] = e < g.ndim ? ts.tile[e] + 1 : 1;
+    ts.slot_elems = ts.lds[0] * ts.lds[1] * ts.lds[2];
+    return ts;
+}
+
+std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slot_capacity,
+                       int group_size, Plan &out) {
+    out = Plan();
+    const std::vector<int32_t> &tg = kind == 0 ? g.odd : g.even;
+    int kaxis_of[3] = {-1, -1, -1};
+    for (int e = 0; e < g.ndim; ++e) kaxis_of[g.axis[e]] = e;
+    int stride[3] = {1, ts.lds[0], ts.lds[0] * ts.lds[1]};
+
+    // sources of every target
+    std::vector<std::vector<int>> src(tg.size());
+    std::set<int> all_src;
+    for (size_t t = 0; t < tg.size(); ++t) {
+        std::set<int> s;
+        for (int d = 0; d < 3; ++d) {
+            const Csr &M = g.M[d];
+            if (M.rp.empty()) continue;
+            for (int p = M.rp[tg[t]]; p < M.rp[tg[t] + 1]; ++p) s.insert(M.ci[p]);
+        }
+        src[t].assign(s.begin(), s.end());
+        all_src.insert(s.begin(), s.end());
+        if ((int)s.size() > slot_capacity) return "a target couples to more components than fit in LDS";
+    }
+    out.n_sources = (int)all_src.size();
+
+    // groups of consecutive targets whose joint sources fit the capacity
+    std::vector<std::pair<int, int>> groups;
+    std::vector<std::vector<int>> gsrc;
+    for (size_t t = 0; t < tg.size();) {
+        size_t t1 = t;
+        std::set<int> s;
+        while (t1 < tg.size() && (int)(t1 - t) < group_size) {
+            std::set<int> s2 = s;
+            s2.insert(src[t1].begin(), src[t1].end());
+            if ((int)s2.size() > slot_capacity) break;
+            s = s2;
+            ++t1;
+        }
+        groups.push_back({(int)t, (int)t1});
+        gsrc.push_back(std::vector<int>(s.begin(), s.end()));
+        t = t1;
+    }
+
+    // slot assignment: keep what the next groups need, evict the source
+    // whose next use is farthest away
+    const int INF = std::numeric_limits<int>::max();
+    std::vector<int> slot_of(g.n_sys, -1);
+    std::vector<int> owner(slot_capacity, -1);
+    auto next_use = [&](int k, size_t from) {
+        for (size_t q = from; q < gsrc.size(); ++q)
+            if (std::binary_search(gsrc[q].begin(), gsrc[q].end(), k)) return (int)q;
+        return INF;
+    };
+    int used = 0;
+    for (size_t q = 0; q < groups.size(); ++q) {
+        DevGroup gr;
+        gr.l0 = (int)out.loads.size();
+        for (int k : gsrc[q]) {
+            if (slot_of[k] >= 0) continue;
+            int s = -1;
+            for (int c = 0; c < slot_capacity; ++c)
+                if (owner[c] < 0) { s = c; break; }
+            if (s < 0) {
+                int best = -1, far = -1;
+                for (int c = 0; c < slot_capacity; ++c) {
+                    int o = owner[c];
+                    if (std::binary_search(gsrc[q].begin(), gsrc[q].end(), o)) continue;
+                    int nu = next_use(o, q + 1);
+                    if (nu > far) { far = nu; best = c; }
+                }
+                if (best < 0) return "LDS slot assignment failed";
+                slot_of[owner[best]] = -1;
+                s = best;
+            }
+            owner[s] = k;
+            slot_of[k] = s;
+            used = std::max(used, s + 1);
+            DevLoad ld;
+            ld.comp = k;
+            ld.slot = s;
+            for (int e = 0; e < 3; ++e) {
+                if (e < g.ndim) {
+                    int ax = g.axis[e];
+                    ld.org[e] = g.gt[3 * k + ax] == 1 ? -1 : 0;
+                    ld.ext[e] = g.ext_of(k, ax);
+                } else {
+                    ld.org[e] = 0;
+                    ld.ext[e] = 1;
+                }
+            }
+            out.loads.push_back(ld);
+        }
+        gr.l1 = (int)out.loads.size();
+        gr.t0 = (int)out.targets.size();
+        for (int t = groups[q].first; t < groups[q].second; ++t) {
+            int j = tg[t];
+            DevTarget dt;
+            dt.comp = j;
+            dt.napp = kind == 0 ? 1 : 2;
+            dt.pad = 0;
+            for (int e = 0; e < 3; ++e) dt.ext[e] = e < g.ndim ? g.ext_of(j, g.axis[e]) : 1;
+            dt.q0 = (int)out.terms.size();
+            // x-terms, then y-terms, then z-terms, ascending k within each:
+            // the summation order of sumcell (starmap_solver.m:350-351, :516-519)
+            for (int d = 0; d < 3; ++d) {
+                const Csr &M = g.M[d];
+                if (M.rp.empty()) continue;
+                int kd = kaxis_of[d];
+                for (int p = M.rp[j]; p < M.rp[j + 1]; ++p) {
+                    int k = M.ci[p];
+                    if (kd < 0) return "coupling along an axis without cells";
+                    DevTerm tm;
+                    int off = slot_of[k] * ts.slot_elems;
+                    for (int e = 0; e < g.ndim; ++e)
+                        if (e != kd && g.gt[3 * k + g.axis[e]] == 1) off += stride[e];
+                    tm.off = off;
+                    tm.stride = stride[kd];
+                    tm.coef = -M.cv[p] / g.h[d];
+                    out.terms.push_back(tm);
+                }
+            }
+            dt.q1 = (int)out.terms.size();
+            out.targets.push_back(dt);
+        }
+        gr.t1 = (int)out.targets.size();
+        out.groups.push_back(gr);
+    }
+    out.n_slots = used;
+    return "";
+}
+
+}  // namespace smp

@@ -1,0 +1,95 @@
+// Host-side planning for the StaRMAP substep kernels (plain C++, no HIP).
+//
+// A substep updates every component of one grid parity ("targets") from
+// difference quotients of the coupled components of the other parity
+// ("sources"), starmap_solver.m:313-359 (odd targets) and :361-414 (even
+// targets).  The device kernel processes one spatial tile per workgroup and
+// walks a schedule of groups: each group first stages the source tiles it
+// needs into LDS slots (tile + one-cell halo), then updates a few targets
+// from LDS.  This file builds that schedule once per problem.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace smp {
+
+// device-visible records (POD, shared with the kernels)
+struct DevGroup {
+    int32_t l0, l1;     // loads [l0,l1)
+    int32_t t0, t1;     // targets [t0,t1)
+};
+
+struct DevLoad {
+    int32_t comp;       // source component
+    int32_t slot;       // LDS slot
+    int32_t org[3];     // tile origin shift per kernel axis (0 or -1)
+    int32_t ext[3];     // source extent per kernel axis (for clamping)
+};
+
+struct DevTerm {
+    int32_t off;        // LDS element offset of the low sample (slot base + shifts)
+    int32_t stride;     // LDS stride of the coupled axis (high = low + stride)
+    double coef;        // -M(j,k)/h
+};
+
+struct DevTarget {
+    int32_t comp;
+    int32_t q0, q1;     // terms [q0,q1)
+    int32_t napp;       // 1 (odd substeps) or 2 (even substep: two half-steps)
+    int32_t ext[3];     // target extent per kernel axis
+    int32_t pad;
+};
+
+struct Csr {
+    std::vector<int32_t> rp, ci;
+    std::vector<double> cv;
+    bool empty() const { return ci.empty(); }
+};
+
+struct Geometry {
+    int n_sys = 0;
+    int n[3] = {1, 1, 1};       // solver axes
+    int bc[3] = {0, 0, 0};
+    double h[3] = {1, 1, 1};
+    Csr M[3];                    // Mx, My, Mz
+    std::vector<int32_t> gt;     // 3*n_sys, values 1/2 (solver axes)
+    int ndim = 0;                // number of kernel axes
+    int axis[3] = {0, 1, 2};     // kernel axis -> solver axis
+    std::vector<int32_t> even, odd;
+    int ext_of(int j, int solver_axis) const {
+        return gt[3 * j + solver_axis] == 1 ? n[solver_axis] : n[solver_axis] + bc[solver_axis];
+    }
+};
+
+// Validate the coupling, compute grid types by two-colouring the coupling
+// graph (equivalent to the fixed-point closure of starmap_solver.m:80-98),
+// choose kernel axes.  Returns "" on success or an error message.
+std::string build_geometry(Geometry &g);
+
+struct TileShape {
+    int tile[3] = {1, 1, 1};     // cells per tile per kernel axis
+    int lds[3] = {1, 1, 1};      // tile + 1 halo per axis
+    int slot_elems = 1;
+    int threads = 256;
+};
+
+TileShape choose_tile(const Geometry &g, int elem_bytes);
+
+struct Plan {
+    std::vector<DevGroup> groups;
+    std::vector<DevLoad> loads;
+    std::vector<DevTarget> targets;
+    std::vector<DevTerm> terms;
+    int n_slots = 0;
+    int n_sources = 0;
+};
+
+// Build the schedule of one substep kind: targets = odd (kind 0) or even
+// (kind 1) components.  slot_capacity bounds the LDS slots; group_size is
+// the number of targets per group.
+std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slot_capacity,
+                       int group_size, Plan &out);
+
+}  // namespace smp

@@ -1,0 +1,76 @@
+"""The C ABI library loads and exports every entry point declared in
+include/starmap_hip.h; host-side logic that needs no GPU."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from starmap_amd import _lib, closure as C, init as I
+from oracle import closure as O
+from oracle import starmap_oracle as so
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "starmap_hip.h")).read()
+    return sorted(set(re.findall(r"\b(smp_[a-z_0-9]+)\s*\(", src)))
+
+
+def test_header_and_binding_agree():
+    assert sorted(_lib.EXPORTS) == _declared()
+
+
+def test_library_exports_every_symbol():
+    L = _lib.lib()
+    for name in _declared():
+        assert hasattr(L, name), name
+    assert b"gfx950" in L.smp_version()
+
+
+def test_create_without_gpu_reports_an_error():
+    """Without a visible device smp_create fails with a status, never a
+    crash; invalid descriptors are rejected before any device call."""
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    d = _lib.Desc()
+    d.n_sys = 0
+    rc = L.smp_create(ctypes.byref(d), ctypes.byref(h))
+    assert rc == _lib.SMP_E_INVALID
+    assert L.smp_last_error()
+    assert L.smp_create(None, ctypes.byref(h)) == _lib.SMP_E_INVALID
+    assert L.smp_step(None, 0.1, 1) == _lib.SMP_E_INVALID
+
+
+@pytest.mark.parametrize("clo", ["closure_pn", "closure_spn"])
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_product_closure_matches_oracle_bitwise(clo, dim):
+    for N in range(1, 16):
+        a = getattr(O, clo)(N, dim)
+        b = getattr(C, clo)(N, dim)
+        for x, y in zip(a, b):
+            assert np.array_equal(np.asarray(x), np.asarray(y))
+
+
+def test_init_lifting_matches_oracle():
+    from starmap_amd import problems as P
+    for prob in (P.gauss_periodic_2d(n=10), P.lattice_2d(n=14, n_mom=5),
+                 P.gauss_periodic_3d(n=6, n_mom=3), P.boxes_2d(n=10, n_mom=3)):
+        a = I.starmap_init(prob)
+        b = so.starmap_init(prob)
+        for k in ("dim", "ax", "n", "bc", "source_ind", "tfinal", "cn", "closure_mod"):
+            assert list(np.atleast_1d(a[k])) == list(np.atleast_1d(b[k])), k
+        for k in ("ic", "sigma_a", "sigma_s0", "source"):
+            assert I.nargin(a[k]) == so.nargin(b[k]), k
+
+
+def test_init_errors_match_reference_messages():
+    with pytest.raises(ValueError, match="Incompatible problem variables ax and n"):
+        I.starmap_init({"closure": "P", "n_mom": 1, "ax": [0, 1, 0, 1], "n": [4]})
+    with pytest.raises(ValueError, match="At least one of ax, n, or bc"):
+        I.starmap_init({"closure": "P", "n_mom": 1})
+    with pytest.raises(ValueError, match="sigma_a has incompatible number of arguments"):
+        I.starmap_init({"closure": "P", "n_mom": 1, "ax": [0, 1, 0, 1],
+                        "sigma_a": lambda x: 0.0})
