@@ -1,0 +1,216 @@
+#!/usr/bin/env python3
+"""Benchmark of the StaRMAP time-stepping core on one MI355X.
+
+Workload (one "step" = one full time step, three Strang substeps, of every
+moment component over the whole grid): 2D P_39 (820 moments) on a 1000x1000
+grid over [0,7]^2, extrapolation boundaries, 7x7 checkerboard with 12 seeded
+absorber blocks (sigma_a=10, sigma_s0=0; elsewhere 0/1), unit isotropic source
+in the centre block, exponential FP_N filter of order 4 and strength 10 at
+substep position, dt = 0.5*dx.  Inputs are resident in HBM before timing.
+
+Prints ONE JSON line: value = FP64 moment-cell updates/s
+(mcu/s = n_sys * cells * steps / wall), the roofline of the substep kernel
+(algorithmic bytes per launch / mean launch time from HIP events on the
+library's stream), the FP32-state variant with its error against the FP64
+state after the same steps, and a CPU baseline (the FP64 oracle on the host
+cores, a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "moment-cell updates/s + % HBM roofline, FP32/batched vs FP64 plain, same L2 err"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def b_alg_per_mcu(n_even, n_odd, n_sys, s):
+    """Compulsory bytes per moment-cell update of the three-substep scheme:
+    substeps 1 and 3 read the even parity and read+write the odd one,
+    substep 2 the reverse."""
+    return s * (4 * n_even + 5 * n_odd) / n_sys
+
+
+def sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def run_gpu(prob, precision, steps, warmup):
+    from starmap_amd import init
+    from starmap_amd.solver import HipSolver
+    s = HipSolver(init.starmap_init(prob), precision=precision)
+    dt = s.prepare()
+    if warmup:
+        s.step(dt, warmup)
+    sync()
+    t0 = time.perf_counter()
+    s.step(dt, steps)
+    sync()
+    wall = time.perf_counter() - t0
+    tot_ms, sub_ms, launches, sub_bytes = s.timing()
+    return s, dt, wall, (sub_ms, launches, sub_bytes)
+
+
+def cpu_baseline(prob, steps):
+    """The FP64 oracle (C restatement of the reference step, OpenMP over
+    components) timed on the host: `steps` full steps of the same workload."""
+    from oracle import starmap_oracle as so
+    par = so.starmap_init(prob)
+    # a final time just past `steps` steps keeps dt = dt_hyp*cn
+    lam = np.max(np.abs(np.linalg.eigvals(par["Mz"])))
+    h = (par["ax"][3] - par["ax"][2]) / par["n"][1]
+    dt = h / lam / 2 * par["cn"]
+    par = dict(par, tfinal=(steps - 0.5) * dt, t_plot=[])
+    _, p2 = so.starmap_solver(par, verbose=False, return_state=True)
+    cpu = p2["_oracle"]["cputime"]
+    return p2["_oracle"]["step_count"], float(cpu[0])
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=1000, help="grid side")
+    ap.add_argument("--n-mom", type=int, default=39)
+    ap.add_argument("--precision", choices=["fp64", "fp32", "both"], default="both")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: the time-stepping core is a single-GPU "
+              "kernel path (--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from starmap_amd import problems as P
+    prob = P.lattice_2d(n=args.n, n_mom=args.n_mom, tfinal=3.2)
+    cells = args.n * args.n
+    res = {}
+    if args.precision in ("fp64", "both"):
+        log("fp64: setting up %dx%d P%d" % (args.n, args.n, args.n_mom))
+        s64, dt, wall, tm = run_gpu(prob, "fp64", args.steps, args.warmup)
+        res["fp64"] = (s64, wall, tm)
+        log("fp64: %d steps in %.3f s" % (args.steps, wall))
+    if args.precision in ("fp32", "both"):
+        log("fp32: setting up")
+        s32, dt, wall, tm = run_gpu(prob, "fp32", args.steps, args.warmup)
+        res["fp32"] = (s32, wall, tm)
+        log("fp32: %d steps in %.3f s" % (args.steps, wall))
+
+    main_key = "fp64" if "fp64" in res else "fp32"
+    s, wall, (sub_ms, launches, sub_bytes) = res[main_key]
+    n_sys = s.n_sys
+    n_even = int(np.sum((s.gt - 1).sum(axis=1) % 2 == 0))
+    n_odd = n_sys - n_even
+    mcu = n_sys * cells * args.steps
+    elem = 8 if main_key == "fp64" else 4
+    bpm = b_alg_per_mcu(n_even, n_odd, n_sys, elem)
+    # the substep kernel's own algorithmic bytes per launch (each staged
+    # source component read once, each target read and written once), over
+    # its mean launch time from HIP events on the library's stream
+    bytes_per_launch = sub_bytes / launches
+    mean_launch_s = sub_ms / 1e3 / launches
+    achieved = bytes_per_launch / mean_launch_s / 1e9
+    out = {
+        "metric": METRIC,
+        "value": mcu / wall,
+        "unit": "moment-cell updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if main_key == "fp64" else "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "2D P%d lattice %dx%d, %s state" % (args.n_mom, args.n, args.n,
+                                                             main_key.upper()),
+            "n_sys": n_sys, "cells": cells, "grid": [args.n, args.n], "bc": "extrapolation",
+            "filter": "exp p=4 f_eff=10 substep", "dt_over_dx": 0.5,
+            "absorber_blocks": 12, "seed": 2024,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "substep_kernel",
+            "bytes_per_launch": bytes_per_launch,
+            "mean_launch_ms": mean_launch_s * 1e3,
+            "launches": launches,
+            # the same throughput priced with the reference's three-substep
+            # splitting (B_alg bytes per moment-cell update)
+            "b_alg_per_mcu": bpm,
+            "b_alg_GBs": bpm * mcu / wall / 1e9,
+            "b_alg_frac": bpm * mcu / wall / 1e9 / HBM_PEAK_GBS,
+        },
+    }
+    if "fp32" in res and "fp64" in res:
+        s32, wall32, (sub32, l32, by32) = res["fp32"]
+        bpm32 = b_alg_per_mcu(n_even, n_odd, n_sys, 4)
+        ach32 = by32 / (sub32 / 1e3) / 1e9
+        # relative L2 per moment field of the FP32 state against the FP64
+        # state after the same steps
+        worst, big = 0.0, 0.0
+        norms, diffs = [], []
+        for j in range(n_sys):
+            a = s64.get_state(j)
+            b = s32.get_state(j)
+            norms.append(np.linalg.norm(a))
+            diffs.append(np.linalg.norm(a - b))
+        norms, diffs = np.array(norms), np.array(diffs)
+        big = norms.max()
+        rel = np.where(norms > 1e-8 * big, diffs / np.maximum(norms, 1e-300), diffs / big)
+        worst = float(rel.max())
+        glob = float(np.sqrt((diffs ** 2).sum() / (norms ** 2).sum()))
+        out["fp32"] = {
+            "value": mcu / wall32,
+            "ms_per_step": wall32 / args.steps * 1e3,
+            "speedup_vs_fp64": (mcu / wall32) / out["value"],
+            "rel_l2_vs_fp64_max_over_moments": worst,
+            "rel_l2_vs_fp64_all_moments": glob,
+            "worst_moment": int(rel.argmax()) + 1,
+            "roofline_achieved_GBs": ach32,
+            "roofline_frac": ach32 / HBM_PEAK_GBS,
+        }
+    for v in res.values():
+        v[0].close()
+
+    if not args.no_cpu_baseline:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        log("cpu baseline: %d steps of the oracle on %d threads" % (args.cpu_steps, threads))
+        nsteps, secs = cpu_baseline(prob, args.cpu_steps)
+        out["cpu_baseline"] = {
+            "value": n_sys * cells * nsteps / secs,
+            "unit": "moment-cell updates/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": "FP64 oracle (C restatement of starmap_solver.m:307-421, OpenMP over "
+                      "components), %d full steps of the same %dx%d P%d workload; stepping "
+                      "loop only, %.1f s" % (nsteps, args.n, args.n, args.n_mom, secs),
+        }
+        out["cpu_baseline"]["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

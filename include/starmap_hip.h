@@ -71,7 +71,10 @@ typedef struct smp_desc {
     const int32_t *f_order;   /* filter order per component, or NULL           */
     int32_t f_position;       /* SMP_FILTER_*                                   */
     int32_t precision;        /* SMP_FP64 or SMP_FP32                           */
-    int32_t device;           /* HIP device ordinal                             */
+    int32_t device;           /* HIP device ordinal; -1 creates a host-only
+                                 handle (validation, grid types and schedules
+                                 without touching a GPU; device calls fail
+                                 with SMP_E_STATE)                             */
 } smp_desc;
 
 /* Last error message of the calling thread ("" if none). */
@@ -134,10 +137,12 @@ int smp_interp(smp_problem *h, double lambda, double *out, int64_t count);
 int smp_observables(smp_problem *h, double *sums, double *sq, int32_t *nonfinite);
 
 /* Timing of the last smp_step call: elapsed milliseconds of all its kernels
- * measured with HIP events on the handle's stream, and the time of the
- * substep kernels alone (sum over launches) with their launch count. */
+ * measured with HIP events on the handle's stream; the time of the substep
+ * kernels alone (sum over the timed launches), their launch count and their
+ * algorithmic HBM bytes (each staged source component read once, each
+ * target component read and written once). */
 int smp_last_step_timing(const smp_problem *h, double *total_ms, double *substep_ms,
-                         int64_t *substep_launches);
+                         int64_t *substep_launches, double *substep_bytes);
 
 /* The HIP stream (hipStream_t) the handle launches on. */
 void *smp_stream(const smp_problem *h);

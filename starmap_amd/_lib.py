@@ -76,7 +76,7 @@ def lib():
         "smp_snapshot": (ctypes.c_int, [vp, vp, i32]),
         "smp_interp": (ctypes.c_int, [vp, dbl, vp, i64]),
         "smp_observables": (ctypes.c_int, [vp, vp, vp, vp]),
-        "smp_last_step_timing": (ctypes.c_int, [vp, vp, vp, vp]),
+        "smp_last_step_timing": (ctypes.c_int, [vp, vp, vp, vp, vp]),
         "smp_stream": (vp, [vp]),
         "smp_device_bytes": (i64, [vp]),
         "smp_plan_stats": (ctypes.c_int, [vp, i32, vp, vp, vp, vp]),

@@ -40,6 +40,12 @@ int fail(int code, const std::string &msg) {
                         std::string(#expr) + ": " + hipGetErrorString(e_));               \
     } while (0)
 
+#define REQUIRE_DEVICE(h)                                                                 \
+    do {                                                                                  \
+        if ((h)->host_only)                                                               \
+            return fail(SMP_E_STATE, "handle was created without a device (device = -1)"); \
+    } while (0)
+
 double expm1div_host(double x) {
     // starmap_solver.m:531-535
     if (std::fabs(x) > 2e-4) return (std::exp(x) - 1) / x;
@@ -72,7 +78,9 @@ struct smp_problem {
     int n_tiles = 0;
     void *U = nullptr;
     void *plan_dev[2] = {nullptr, nullptr};
+    size_t plan_off[2][3] = {{0, 0, 0}, {0, 0, 0}};   // loads, targets offsets
     size_t lds_bytes[2] = {0, 0};
+    bool host_only = false;
     std::vector<int32_t> mom_order, f_order;
     Field sA1, EA;
     std::map<Key, Field> sT, ET, F;
@@ -80,14 +88,13 @@ struct smp_problem {
     double dt_decay = std::nan("");
     bool decay_dirty = true;
     bool coef_dirty = true;
-    void *coefs = nullptr;
     std::vector<int32_t> snap_comps;
     void *snap = nullptr;
     void *interp_buf = nullptr;
     int64_t interp_cap = 0;
     void *red_buf = nullptr;
     std::vector<hipEvent_t> events;
-    double last_total_ms = 0, last_substep_ms = 0;
+    double last_total_ms = 0, last_substep_ms = 0, last_bytes = 0;
     int64_t last_launches = 0;
     int64_t dev_bytes = 0;
     std::vector<void *> allocs;
@@ -246,11 +253,14 @@ int smp_create(const smp_desc *d, smp_problem **out) {
     if (d->f_order) h->f_order.assign(d->f_order, d->f_order + d->n_sys);
     h->Q.resize(d->n_sys);
 
-    int ndev = 0;
-    HIP_TRY(hipGetDeviceCount(&ndev));
-    if (d->device < 0 || d->device >= ndev) return fail(SMP_E_INVALID, "no such HIP device");
-    HIP_TRY(hipSetDevice(d->device));
-    HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    h->host_only = d->device < 0;
+    if (!h->host_only) {
+        int ndev = 0;
+        HIP_TRY(hipGetDeviceCount(&ndev));
+        if (d->device >= ndev) return fail(SMP_E_INVALID, "no such HIP device");
+        HIP_TRY(hipSetDevice(d->device));
+        HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    }
 
     // storage geometry
     for (int e = 0; e < 3; ++e) {
@@ -272,39 +282,59 @@ int smp_create(const smp_desc *d, smp_problem **out) {
         h->n_tiles *= h->ntile[e];
     }
     // schedules
-    const int lds_budget = 160 * 1024;
-    const int cap = lds_budget / (h->ts.slot_elems * h->elem);
+    const size_t lds_budget = 160 * 1024 - smp::substep_lds_overhead();
+    const int cap = (int)(lds_budget / ((size_t)h->ts.slot_elems * h->elem));
     for (int kind = 0; kind < 2; ++kind) {
-        err = smp::build_plan(h->geo, h->ts, kind, cap, 4, h->plan[kind]);
+        err = smp::build_plan(h->geo, h->ts, kind, cap, smp::GMAX, smp::PF * h->ts.threads,
+                              h->plan[kind]);
         if (!err.empty()) return fail(SMP_E_INVALID, err);
+        // the smallest slot count that still stages every source once per
+        // tile: less LDS per workgroup, more workgroups per CU
+        if ((int)h->plan[kind].loads.size() == h->plan[kind].n_sources) {
+            int lo = 1, hi = h->plan[kind].n_slots;
+            while (lo < hi) {
+                const int mid = (lo + hi) / 2;
+                smp::Plan trial;
+                std::string e2 = smp::build_plan(h->geo, h->ts, kind, mid, smp::GMAX,
+                                                 smp::PF * h->ts.threads, trial);
+                if (e2.empty() && (int)trial.loads.size() == trial.n_sources)
+                    hi = mid;
+                else
+                    lo = mid + 1;
+            }
+            err = smp::build_plan(h->geo, h->ts, kind, lo, smp::GMAX, smp::PF * h->ts.threads,
+                                  h->plan[kind]);
+            if (!err.empty()) return fail(SMP_E_INVALID, err);
+        }
         const smp::Plan &p = h->plan[kind];
-        h->lds_bytes[kind] = (size_t)std::max(p.n_slots, 1) * h->ts.slot_elems * h->elem;
-        // device copy: groups | loads | targets | terms, each 16-byte aligned
+        if ((int)p.groups.size() > smp::MAX_GROUPS)
+            return fail(SMP_E_INVALID, "substep schedule has too many groups");
+        h->lds_bytes[kind] = smp::substep_lds_overhead() +
+                             (size_t)std::max(p.n_slots, 1) * h->ts.slot_elems * h->elem;
+        auto al = [](size_t v) { return (v + 255) / 256 * 256; };
         size_t s0 = p.groups.size() * sizeof(smp::DevGroup);
         size_t s1 = p.loads.size() * sizeof(smp::DevLoad);
-        size_t s2 = p.targets.size() * sizeof(smp::DevTarget);
-        size_t s3 = p.terms.size() * sizeof(smp::DevTerm);
-        auto al = [](size_t v) { return (v + 15) / 16 * 16; };
-        size_t bytes = al(s0) + al(s1) + al(s2) + al(s3) + 16;
-        int rc = h->dalloc(&h->plan_dev[kind], bytes);
+        size_t s2 = p.targets.size() * sizeof(smp::TgtRec);
+        h->plan_off[kind][0] = al(s0);
+        h->plan_off[kind][1] = al(s0) + al(s1);
+        h->plan_off[kind][2] = al(s0) + al(s1) + al(s2);
+        if (h->host_only) continue;
+        int rc = h->dalloc(&h->plan_dev[kind], h->plan_off[kind][2]);
         if (rc) return rc;
-        std::vector<unsigned char> buf(bytes, 0);
-        size_t o = 0;
-        std::memcpy(buf.data() + o, p.groups.data(), s0);
-        o += al(s0);
-        std::memcpy(buf.data() + o, p.loads.data(), s1);
-        o += al(s1);
-        std::memcpy(buf.data() + o, p.targets.data(), s2);
-        o += al(s2);
-        std::memcpy(buf.data() + o, p.terms.data(), s3);
-        HIP_TRY(hipMemcpy(h->plan_dev[kind], buf.data(), bytes, hipMemcpyHostToDevice));
+        std::vector<unsigned char> buf(h->plan_off[kind][2], 0);
+        std::memcpy(buf.data(), p.groups.data(), s0);
+        std::memcpy(buf.data() + h->plan_off[kind][0], p.loads.data(), s1);
+        std::memcpy(buf.data() + h->plan_off[kind][1], p.targets.data(), s2);
+        HIP_TRY(hipMemcpy(h->plan_dev[kind], buf.data(), buf.size(), hipMemcpyHostToDevice));
+    }
+    if (h->host_only) {
+        *out = h.release();
+        return SMP_OK;
     }
     // state
     int rc = h->dalloc(&h->U, (size_t)h->slab * h->elem * d->n_sys);
     if (rc) return rc;
     HIP_TRY(hipMemsetAsync(h->U, 0, (size_t)h->slab * h->elem * d->n_sys, h->stream));
-    rc = h->dalloc(&h->coefs, sizeof(smp::CoefRec<double>) * d->n_sys);
-    if (rc) return rc;
     rc = h->dalloc(&h->red_buf, sizeof(double) * (3 * 1024 + 3));
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(h->stream));
@@ -314,8 +344,10 @@ int smp_create(const smp_desc *d, smp_problem **out) {
 
 int smp_destroy(smp_problem *h) {
     if (!h) return SMP_OK;
-    (void)hipSetDevice(h->device);
-    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (!h->host_only) {
+        (void)hipSetDevice(h->device);
+        if (h->stream) (void)hipStreamSynchronize(h->stream);
+    }
     delete h;
     return SMP_OK;
 }
@@ -335,6 +367,7 @@ int smp_component_shape(const smp_problem *h, int32_t j, int64_t shape[3]) {
 
 int smp_set_state(smp_problem *h, int32_t j, const double *src, int64_t count) {
     if (!h || !src) return fail(SMP_E_INVALID, "null argument");
+    REQUIRE_DEVICE(h);
     if (j < 0 || j >= h->geo.n_sys) return fail(SMP_E_INVALID, "component out of range");
     if (count != h->numel(j)) return fail(SMP_E_INVALID, "count does not match the component shape");
     HIP_TRY(hipSetDevice(h->device));
@@ -355,6 +388,7 @@ int smp_set_state(smp_problem *h, int32_t j, const double *src, int64_t count) {
 
 int smp_get_state(smp_problem *h, int32_t j, double *dst, int64_t count) {
     if (!h || !dst) return fail(SMP_E_INVALID, "null argument");
+    REQUIRE_DEVICE(h);
     if (j < 0 || j >= h->geo.n_sys) return fail(SMP_E_INVALID, "component out of range");
     if (count != h->numel(j)) return fail(SMP_E_INVALID, "count does not match the component shape");
     HIP_TRY(hipSetDevice(h->device));
@@ -379,6 +413,7 @@ int smp_set_coeff(smp_problem *h, int32_t kind, const int32_t gt_in[3], int32_t 
     HIP_TRY(hipSetDevice(h->device));
     HIP_TRY(hipStreamSynchronize(h->stream));
     int gt[3] = {1, 1, 1};
+    REQUIRE_DEVICE(h);
     if (kind == SMP_COEF_SIGMA_T || kind == SMP_COEF_FILTER) {
         if (!gt_in) return fail(SMP_E_INVALID, "grid type required");
         for (int d = 0; d < 3; ++d) {
@@ -479,9 +514,10 @@ int refresh_decay(smp_problem *h, double dt) {
 template <typename T>
 int refresh_coefs(smp_problem *h) {
     const int n = h->geo.n_sys;
-    std::vector<smp::CoefRec<T>> rec(n);
+    // per-component coefficient part of the target records
+    std::vector<smp::TgtRec> cf(n);
     for (int j = 0; j < n; ++j) {
-        smp::CoefRec<T> &c = rec[j];
+        smp::TgtRec &c = cf[j];
         std::memset(&c, 0, sizeof(c));
         const Field *sig, *E;
         if (j == 0) {
@@ -497,11 +533,11 @@ int refresh_coefs(smp_problem *h) {
             sig = &it->second;
             E = &h->ET[k];
         }
-        c.sig = (const T *)sig->dev;
+        c.sig = sig->dev;
         c.sig_s = sig->scalar;
-        c.E = (const T *)E->dev;
+        c.E = E->dev;
         c.E_s = E->scalar;
-        c.Q = h->Q[j].set ? (const T *)h->Q[j].dev : nullptr;
+        c.Q = h->Q[j].set ? h->Q[j].dev : nullptr;
         c.F_s[0] = c.F_s[1] = 1.0;
         c.filt = 0;
         if (j > 0 && h->f_position == SMP_FILTER_SUBSTEP) {
@@ -511,7 +547,7 @@ int refresh_coefs(smp_problem *h) {
                 auto it = h->F.find(k);
                 if (it == h->F.end() || !it->second.set)
                     return fail(SMP_E_STATE, "substep filter not set for component " + std::to_string(j));
-                c.F[s] = (const T *)it->second.dev;
+                c.F[s] = it->second.dev;
                 c.F_s[s] = it->second.scalar;
             }
         } else if (j > 0 && h->f_position == SMP_FILTER_FULL) {
@@ -519,17 +555,32 @@ int refresh_coefs(smp_problem *h) {
             auto it = h->F.find(k);
             if (it == h->F.end() || !it->second.set)
                 return fail(SMP_E_STATE, "full filter not set for component " + std::to_string(j));
-            c.F[0] = (const T *)it->second.dev;
-            c.F_s[0] = it->second.scalar;
         }
     }
-    HIP_TRY(hipMemcpy(h->coefs, rec.data(), sizeof(smp::CoefRec<T>) * n, hipMemcpyHostToDevice));
+    for (int kind = 0; kind < 2; ++kind) {
+        std::vector<smp::TgtRec> recs = h->plan[kind].targets;
+        for (smp::TgtRec &r : recs) {
+            const smp::TgtRec &c = cf[r.comp];
+            r.filt = c.filt;
+            r.sig = c.sig;
+            r.E = c.E;
+            r.Q = c.Q;
+            r.F[0] = c.F[0];
+            r.F[1] = c.F[1];
+            r.sig_s = c.sig_s;
+            r.E_s = c.E_s;
+            r.F_s[0] = c.F_s[0];
+            r.F_s[1] = c.F_s[1];
+        }
+        HIP_TRY(hipMemcpy((char *)h->plan_dev[kind] + h->plan_off[kind][1], recs.data(),
+                          recs.size() * sizeof(smp::TgtRec), hipMemcpyHostToDevice));
+    }
     h->coef_dirty = false;
     return SMP_OK;
 }
 
 template <typename T>
-smp::SubstepArgs<T> make_args(smp_problem *h, int kind, double dt, int fslot) {
+smp::SubstepArgs<T> make_args(smp_problem *h, int kind, double dt, int napp, int fs0, int fs1) {
     smp::SubstepArgs<T> a;
     std::memset(&a, 0, sizeof(a));
     a.U = (T *)h->U;
@@ -551,19 +602,15 @@ smp::SubstepArgs<T> make_args(smp_problem *h, int kind, double dt, int fslot) {
     a.slot_elems = h->ts.slot_elems;
     a.n_tiles = h->n_tiles;
     const smp::Plan &p = h->plan[kind];
-    auto al = [](size_t v) { return (v + 15) / 16 * 16; };
     char *base = (char *)h->plan_dev[kind];
-    size_t s0 = p.groups.size() * sizeof(smp::DevGroup);
-    size_t s1 = p.loads.size() * sizeof(smp::DevLoad);
-    size_t s2 = p.targets.size() * sizeof(smp::DevTarget);
     a.groups = (const smp::DevGroup *)base;
-    a.loads = (const smp::DevLoad *)(base + al(s0));
-    a.targets = (const smp::DevTarget *)(base + al(s0) + al(s1));
-    a.terms = (const smp::DevTerm *)(base + al(s0) + al(s1) + al(s2));
+    a.loads = (const smp::DevLoad *)(base + h->plan_off[kind][0]);
+    a.targets = (const smp::TgtRec *)(base + h->plan_off[kind][1]);
     a.ngroups = (int)p.groups.size();
-    a.coefs = (const smp::CoefRec<T> *)h->coefs;
     a.half_dt = dt / 2;
-    a.fslot = fslot;
+    a.napp = napp;
+    a.fslot[0] = fs0;
+    a.fslot[1] = fs1;
     return a;
 }
 
@@ -577,15 +624,32 @@ int run_steps(smp_problem *h, double dt, int64_t n_steps) {
         int rc = refresh_coefs<T>(h);
         if (rc) return rc;
     }
-    smp::SubstepArgs<T> odd1 = make_args<T>(h, 0, dt, 0);
-    smp::SubstepArgs<T> even = make_args<T>(h, 1, dt, 0);
-    smp::SubstepArgs<T> odd3 = make_args<T>(h, 0, dt, 1);
-    const int per_step = 3;
-    const int64_t launches = n_steps * per_step;
+    // Launch sequence for n steps.  Substep 3 of a step and substep 1 of the
+    // next one update the odd components twice from the same even state,
+    // i.e. with the same flux W: inside one call they run as one launch with
+    // two applications (filter slot k=2, then k=1), bit-identical to two
+    // launches.  n steps take 2n+1 launches.
+    const smp::SubstepArgs<T> odd_first = make_args<T>(h, 0, dt, 1, 0, 0);
+    const smp::SubstepArgs<T> even = make_args<T>(h, 1, dt, 2, 0, 1);
+    const smp::SubstepArgs<T> odd_fused = make_args<T>(h, 0, dt, 2, 1, 0);
+    const smp::SubstepArgs<T> odd_last = make_args<T>(h, 0, dt, 1, 1, 1);
+    const bool full = h->f_position == SMP_FILTER_FULL;
+    std::vector<const smp::SubstepArgs<T> *> seq;
+    seq.reserve(2 * n_steps + 1);
+    for (int64_t s = 0; s < n_steps; ++s) {
+        if (s == 0 || full)
+            seq.push_back(&odd_first);
+        else
+            seq.push_back(&odd_fused);
+        seq.push_back(&even);
+        if (full || s == n_steps - 1) seq.push_back(&odd_last);
+        if (full) seq.push_back(nullptr);   // full filter after the step
+    }
+    int64_t n_sub = 0;
+    for (auto *p : seq) n_sub += p != nullptr;
     // a start/end event pair around every substep launch (up to a cap):
     // their differences are the substep kernel durations on this stream
-    const int64_t timed = std::min<int64_t>(launches, 3 * 4096);
-    // two extra events bracket the whole call
+    const int64_t timed = std::min<int64_t>(n_sub, 3 * 4096);
     while ((int64_t)h->events.size() < 2 * timed + 2) {
         hipEvent_t e;
         HIP_TRY(hipEventCreate(&e));
@@ -594,23 +658,21 @@ int run_steps(smp_problem *h, double dt, int64_t n_steps) {
     hipEvent_t ev_start = h->events[0], ev_end = h->events[1];
     HIP_TRY(hipEventRecord(ev_start, h->stream));
     int64_t li = 0;
-    for (int64_t s = 0; s < n_steps; ++s) {
-        const smp::SubstepArgs<T> *seq[3] = {&odd1, &even, &odd3};
-        for (int k = 0; k < 3; ++k) {
-            const int kind = k == 1 ? 1 : 0;
-            if (li < timed) HIP_TRY(hipEventRecord(h->events[2 + 2 * li], h->stream));
-            HIP_TRY(smp::launch_substep<T>(*seq[k], h->ts.threads, h->lds_bytes[kind], h->stream));
-            if (li < timed) HIP_TRY(hipEventRecord(h->events[3 + 2 * li], h->stream));
-            ++li;
-        }
-        if (h->f_position == SMP_FILTER_FULL) {
+    for (auto *p : seq) {
+        if (!p) {
             for (int j = 1; j < h->geo.n_sys; ++j) {
                 Key k(h->geo.gt[3 * j], h->geo.gt[3 * j + 1], h->geo.gt[3 * j + 2], h->f_order[j], 0);
                 const Field &f = h->F[k];
                 HIP_TRY(smp::launch_scale<T>((T *)h->U + (size_t)j * h->slab, (const T *)f.dev, f.scalar,
                                              h->slab, h->stream));
             }
+            continue;
         }
+        const int kind = p == &even ? 1 : 0;
+        if (li < timed) HIP_TRY(hipEventRecord(h->events[2 + 2 * li], h->stream));
+        HIP_TRY(smp::launch_substep<T>(*p, h->ts.threads, h->lds_bytes[kind], h->stream));
+        if (li < timed) HIP_TRY(hipEventRecord(h->events[3 + 2 * li], h->stream));
+        ++li;
     }
     HIP_TRY(hipEventRecord(ev_end, h->stream));
     HIP_TRY(hipEventSynchronize(ev_end));
@@ -625,6 +687,20 @@ int run_steps(smp_problem *h, double dt, int64_t n_steps) {
     }
     h->last_substep_ms = sub_ms;
     h->last_launches = timed;
+    h->last_bytes = 0;
+    // algorithmic bytes of the timed launches: every staged source component
+    // read once, every target read and written once
+    {
+        const double elem = (double)h->elem;
+        const double cells = (double)h->geo.n[0] * h->geo.n[1] * h->geo.n[2];
+        const double ne = (double)h->geo.even.size(), no = (double)h->geo.odd.size();
+        int64_t c = 0;
+        for (auto *p : seq) {
+            if (!p) continue;
+            if (c++ >= timed) break;
+            h->last_bytes += p == &even ? elem * cells * (no + 2 * ne) : elem * cells * (ne + 2 * no);
+        }
+    }
     return SMP_OK;
 }
 
@@ -636,6 +712,7 @@ int smp_step(smp_problem *h, double dt, int64_t n_steps) {
     if (!h) return fail(SMP_E_INVALID, "null handle");
     if (!(dt > 0) || !std::isfinite(dt)) return fail(SMP_E_INVALID, "dt must be positive and finite");
     if (n_steps < 0) return fail(SMP_E_INVALID, "n_steps must be >= 0");
+    REQUIRE_DEVICE(h);
     HIP_TRY(hipSetDevice(h->device));
     if (n_steps == 0) return SMP_OK;
     return h->precision == SMP_FP64 ? run_steps<double>(h, dt, n_steps)
@@ -647,6 +724,7 @@ int smp_snapshot(smp_problem *h, const int32_t *comps, int32_t n_comps) {
     if (n_comps < 0) return fail(SMP_E_INVALID, "n_comps must be >= 0");
     for (int i = 0; i < n_comps; ++i)
         if (comps[i] < 0 || comps[i] >= h->geo.n_sys) return fail(SMP_E_INVALID, "component out of range");
+    REQUIRE_DEVICE(h);
     HIP_TRY(hipSetDevice(h->device));
     if ((int)h->snap_comps.size() != n_comps) {
         h->dfree(h->snap);
@@ -667,6 +745,7 @@ int smp_snapshot(smp_problem *h, const int32_t *comps, int32_t n_comps) {
 
 int smp_interp(smp_problem *h, double lambda, double *out, int64_t count) {
     if (!h || !out) return fail(SMP_E_INVALID, "null argument");
+    REQUIRE_DEVICE(h);
     if (h->snap_comps.empty()) return fail(SMP_E_STATE, "no snapshot taken");
     int64_t total = 0;
     for (int j : h->snap_comps) total += h->numel(j);
@@ -704,6 +783,7 @@ int smp_interp(smp_problem *h, double lambda, double *out, int64_t count) {
 
 int smp_observables(smp_problem *h, double *sums, double *sq, int32_t *nonfinite) {
     if (!h) return fail(SMP_E_INVALID, "null handle");
+    REQUIRE_DEVICE(h);
     HIP_TRY(hipSetDevice(h->device));
     const int blocks = 1024;
     double *part = (double *)h->red_buf;
@@ -746,8 +826,9 @@ int smp_observables(smp_problem *h, double *sums, double *sq, int32_t *nonfinite
 }
 
 int smp_last_step_timing(const smp_problem *h, double *total_ms, double *substep_ms,
-                         int64_t *substep_launches) {
+                         int64_t *substep_launches, double *substep_bytes) {
     if (!h) return fail(SMP_E_INVALID, "null handle");
+    if (substep_bytes) *substep_bytes = h->last_bytes;
     if (total_ms) *total_ms = h->last_total_ms;
     if (substep_ms) *substep_ms = h->last_substep_ms;
     if (substep_launches) *substep_launches = h->last_launches;

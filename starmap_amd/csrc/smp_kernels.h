@@ -7,19 +7,9 @@
 
 namespace smp {
 
-// Per-component coefficients of one half-step update.  A null field pointer
-// means the matching scalar applies everywhere.
-template <typename T>
-struct CoefRec {
-    const T *sig;     // decay rate: sA1 (j=0) or sT1{gt,order}
-    const T *E;       // expm1div(-sig*dt/2)
-    const T *Q;       // source, or null for Q = 0
-    const T *F[2];    // substep filter slots k=1,2 (or full filter in F[0])
-    double sig_s, E_s;
-    double F_s[2];
-    int32_t filt;     // apply substep filter after each application
-    int32_t pad;
-};
+constexpr int GMAX = 8;       // targets per group
+constexpr int PF = 16;        // staged source elements per thread per group
+constexpr int MAX_GROUPS = 256;
 
 template <typename T>
 struct SubstepArgs {
@@ -33,12 +23,14 @@ struct SubstepArgs {
     int32_t ngroups;
     const DevGroup *groups;
     const DevLoad *loads;
-    const DevTarget *targets;
-    const DevTerm *terms;
-    const CoefRec<T> *coefs;
+    const TgtRec *targets;
     double half_dt;
-    int32_t fslot;            // filter slot of single-application targets
+    int32_t napp;             // half-step applications with the same flux (1 or 2)
+    int32_t fslot[2];         // filter slot of application 0 and 1
 };
+
+// LDS bytes the kernel needs besides the source slots
+size_t substep_lds_overhead();
 
 template <typename T>
 hipError_t launch_substep(const SubstepArgs<T> &a, int threads, size_t lds_bytes,

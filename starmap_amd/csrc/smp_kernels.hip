@@ -40,13 +40,26 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 
 }  // namespace
 
+size_t substep_lds_overhead() {
+    return 2 * GMAX * sizeof(TgtRec) + MAX_GROUPS * sizeof(DevGroup);
+}
+
+// One workgroup = one spatial tile, one thread = one cell.  LDS holds the
+// group headers, two buffers of target records and the source slots.  While
+// group g is updated from LDS, the source tiles and target values of group
+// g+1 and the records of group g+2 are in flight in registers.
 template <typename T>
 __global__ __launch_bounds__(256) void substep_kernel(const SubstepArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T *lds = reinterpret_cast<T *>(smem_raw);
+    TgtRec *rec = reinterpret_cast<TgtRec *>(smem_raw);
+    DevGroup *grp = reinterpret_cast<DevGroup *>(smem_raw + 2 * GMAX * sizeof(TgtRec));
+    T *lds = reinterpret_cast<T *>(smem_raw + 2 * GMAX * sizeof(TgtRec) +
+                                   MAX_GROUPS * sizeof(DevGroup));
+    constexpr int RECV = sizeof(TgtRec) / 16;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= a.n_tiles) return;
     const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
     int b = tile;
     const int t0 = b % a.ntile[0];
     b /= a.ntile[0];
@@ -59,57 +72,127 @@ __global__ __launch_bounds__(256) void substep_kernel(const SubstepArgs<T> a) {
     const int g0 = o0 + c0, g1 = o1 + c1, g2 = o2 + c2;
     const int lbase = c0 + a.lds[0] * (c1 + a.lds[1] * c2);
     const int64_t gidx = g0 + (int64_t)a.pitch0 * (g1 + (int64_t)a.ext1 * g2);
+    const int S = a.slot_elems;
     const int L0 = a.lds[0], L1 = a.lds[1];
     const T half_dt = (T)a.half_dt;
+    const int ng = a.ngroups;
 
-    for (int gq = 0; gq < a.ngroups; ++gq) {
-        const DevGroup gr = a.groups[gq];
-        // stage the group's new source tiles (tile + one-cell halo) in LDS
-        for (int l = gr.l0; l < gr.l1; ++l) {
-            const DevLoad ld = a.loads[l];
-            const T *src = a.U + (int64_t)ld.comp * a.slab;
-            T *dst = lds + ld.slot * a.slot_elems;
-            for (int e = tid; e < a.slot_elems; e += blockDim.x) {
-                const int l0 = e % L0;
-                const int r = e / L0;
-                const int l1 = r % L1;
-                const int l2 = r / L1;
+    for (int i = tid; i < ng; i += nthr) grp[i] = a.groups[i];
+
+    T pf[PF];
+    int pdst[PF];
+    int4 rpf = make_int4(0, 0, 0, 0);
+    T un_cur[GMAX], un_nxt[GMAX];
+
+    // global -> registers: the source tiles of group gq
+    auto fetch_src = [&](const DevGroup &gr) {
+        const int total = (gr.l1 - gr.l0) * S;
+#pragma unroll
+        for (int k = 0; k < PF; ++k) {
+            const int e = tid + k * nthr;
+            pdst[k] = -1;
+            if (e < total) {
+                const int li = e / S;
+                const int r = e - li * S;
+                const DevLoad ld = a.loads[gr.l0 + li];
+                const int l0 = r % L0;
+                const int rr = r / L0;
+                const int l1 = rr % L1;
+                const int l2 = rr / L1;
                 const int x0 = wrap_or_clamp(o0 + l0 + ld.org[0], a.n[0], a.bc[0], ld.ext[0]);
                 const int x1 = wrap_or_clamp(o1 + l1 + ld.org[1], a.n[1], a.bc[1], ld.ext[1]);
                 const int x2 = wrap_or_clamp(o2 + l2 + ld.org[2], a.n[2], a.bc[2], ld.ext[2]);
-                dst[e] = src[x0 + (int64_t)a.pitch0 * (x1 + (int64_t)a.ext1 * x2)];
+                pf[k] = a.U[(int64_t)ld.comp * a.slab + x0 +
+                            (int64_t)a.pitch0 * (x1 + (int64_t)a.ext1 * x2)];
+                pdst[k] = ld.slot * S + r;
             }
         }
-        __syncthreads();
-        for (int t = gr.t0; t < gr.t1; ++t) {
-            const DevTarget tg = a.targets[t];
-            if (g0 < tg.ext[0] && g1 < tg.ext[1] && g2 < tg.ext[2]) {
-                // W = -sumcell(...): the coefficients carry -M(j,k)/h
-                T w = 0;
-                for (int q = tg.q0; q < tg.q1; ++q) {
-                    const DevTerm tm = a.terms[q];
-                    const T lo = lds[tm.off + lbase];
-                    const T hi = lds[tm.off + lbase + tm.stride];
-                    w += (T)tm.coef * (hi - lo);
-                }
-                const CoefRec<T> &c = a.coefs[tg.comp];
-                T *up = a.U + (int64_t)tg.comp * a.slab + gidx;
-                T u = *up;
-                const T sig = c.sig ? c.sig[gidx] : (T)c.sig_s;
-                const T E = c.E ? c.E[gidx] : (T)c.E_s;
-                const T wq = c.Q ? w + c.Q[gidx] : w;
-                for (int app = 0; app < tg.napp; ++app) {
-                    // U + dt/2*(W+Q - s.*U).*E   (starmap_solver.m:352-354)
-                    u = u + (half_dt * (wq - sig * u)) * E;
-                    if (c.filt) {
-                        const int s = tg.napp == 2 ? app : a.fslot;
-                        u = u * (c.F[s] ? c.F[s][gidx] : (T)c.F_s[s]);
+    };
+    auto commit_src = [&]() {
+#pragma unroll
+        for (int k = 0; k < PF; ++k)
+            if (pdst[k] >= 0) lds[pdst[k]] = pf[k];
+    };
+    // global -> registers -> LDS: the target records of a group
+    auto fetch_rec = [&](const DevGroup &gr) {
+        if (tid < (gr.t1 - gr.t0) * RECV)
+            rpf = reinterpret_cast<const int4 *>(a.targets + gr.t0)[tid];
+    };
+    auto commit_rec = [&](const DevGroup &gr, int buf) {
+        if (tid < (gr.t1 - gr.t0) * RECV)
+            reinterpret_cast<int4 *>(rec + buf * GMAX)[tid] = rpf;
+    };
+    auto load_targets = [&](const DevGroup &gr, int buf, T *dst) {
+        const TgtRec *r = rec + buf * GMAX;
+#pragma unroll
+        for (int i = 0; i < GMAX; ++i) {
+            if (i < gr.t1 - gr.t0 && g0 < r[i].ext[0] && g1 < r[i].ext[1] && g2 < r[i].ext[2])
+                dst[i] = a.U[(int64_t)r[i].comp * a.slab + gidx];
+        }
+    };
+
+    __syncthreads();
+    if (ng > 0) {
+        fetch_src(grp[0]);
+        fetch_rec(grp[0]);
+        commit_src();
+        commit_rec(grp[0], 0);
+        if (ng > 1) {
+            fetch_rec(grp[1]);
+            commit_rec(grp[1], 1);
+        }
+    }
+    __syncthreads();
+    if (ng > 0) load_targets(grp[0], 0, un_cur);
+
+    for (int gq = 0; gq < ng; ++gq) {
+        const DevGroup gr = grp[gq];
+        const int buf = gq & 1;
+        if (gq + 1 < ng) {
+            load_targets(grp[gq + 1], buf ^ 1, un_nxt);
+            fetch_src(grp[gq + 1]);
+        }
+        if (gq + 2 < ng) fetch_rec(grp[gq + 2]);
+        const TgtRec *rb = rec + buf * GMAX;
+        const int nt = gr.t1 - gr.t0;
+#pragma unroll
+        for (int i = 0; i < GMAX; ++i) {
+            if (i < nt) {
+                const TgtRec &r = rb[i];
+                if (g0 < r.ext[0] && g1 < r.ext[1] && g2 < r.ext[2]) {
+                    // W = -sumcell(...): the coefficients carry -M(j,k)/h
+                    T w = 0;
+                    const int nterm = r.nterm;
+#pragma unroll
+                    for (int q = 0; q < MAXT; ++q) {
+                        if (q < nterm) {
+                            const int o = r.off[q] + lbase;
+                            w += (T)r.coef[q] * (lds[o + r.stride[q]] - lds[o]);
+                        }
                     }
+                    const T sig = r.sig ? reinterpret_cast<const T *>(r.sig)[gidx] : (T)r.sig_s;
+                    const T E = r.E ? reinterpret_cast<const T *>(r.E)[gidx] : (T)r.E_s;
+                    const T wq = r.Q ? w + reinterpret_cast<const T *>(r.Q)[gidx] : w;
+                    T u = un_cur[i];
+                    for (int app = 0; app < a.napp; ++app) {
+                        // U + dt/2*(W+Q - s.*U).*E   (starmap_solver.m:352-354)
+                        u = u + (half_dt * (wq - sig * u)) * E;
+                        if (r.filt) {
+                            const int s = a.fslot[app];
+                            u = u * (r.F[s] ? reinterpret_cast<const T *>(r.F[s])[gidx]
+                                            : (T)r.F_s[s]);
+                        }
+                    }
+                    a.U[(int64_t)r.comp * a.slab + gidx] = u;
                 }
-                *up = u;
             }
         }
         __syncthreads();
+        if (gq + 1 < ng) commit_src();
+        if (gq + 2 < ng) commit_rec(grp[gq + 2], buf);
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < GMAX; ++i) un_cur[i] = un_nxt[i];
     }
 }
 
@@ -124,6 +207,7 @@ hipError_t launch_substep(const SubstepArgs<T> &a, int threads, size_t lds_bytes
         if (e != hipSuccess) return e;
         attr_set = true;
     }
+    if (a.ngroups > MAX_GROUPS) return hipErrorInvalidValue;
     const int nb = ((a.n_tiles + 7) / 8) * 8;
     hipLaunchKernelGGL(substep_kernel<T>, dim3(nb), dim3(threads), lds_bytes, s, a);
     return hipGetLastError();

@@ -2,6 +2,7 @@
 #include "smp_plan.h"
 
 #include <algorithm>
+#include <cstring>
 #include <deque>
 #include <limits>
 #include <set>
@@ -84,43 +85,46 @@ std::string build_geometry(Geometry &g) {
     return "";
 }
 
-TileShape choose_tile(const Geometry &g, int elem_bytes) {
-    (void)elem_bytes;
+TileShape make_tile(const Geometry &g, int tx, int ty, int tz) {
     TileShape ts;
-    if (g.ndim == 1) {
-        ts.tile[0] = 256;
-    } else if (g.ndim == 2) {
-        ts.tile[0] = 64;
-        ts.tile[1] = 4;
-    } else {
-        ts.tile[0] = 32;
-        ts.tile[1] = 4;
-        ts.tile[2] = 2;
-    }
+    int t[3] = {tx, ty, tz};
+    for (int e = 0; e < 3; ++e) ts.tile[e] = e < g.ndim ? t[e] : 1;
     ts.threads = ts.tile[0] * ts.tile[1] * ts.tile[2];
     for (int e = 0; e < 3; ++e) ts.lds[e] = e < g.ndim ? ts.tile[e] + 1 : 1;
     ts.slot_elems = ts.lds[0] * ts.lds[1] * ts.lds[2];
     return ts;
 }
 
+TileShape choose_tile(const Geometry &g, int elem_bytes) {
+    (void)elem_bytes;
+    if (g.ndim == 1) return make_tile(g, 256, 1, 1);
+    if (g.ndim == 2) return make_tile(g, 64, 4, 1);
+    return make_tile(g, 32, 4, 2);
+}
+
 std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slot_capacity,
-                       int group_size, Plan &out) {
+                       int group_size, int max_load_elems, Plan &out) {
     out = Plan();
     const std::vector<int32_t> &tg = kind == 0 ? g.odd : g.even;
     int kaxis_of[3] = {-1, -1, -1};
     for (int e = 0; e < g.ndim; ++e) kaxis_of[g.axis[e]] = e;
-    int stride[3] = {1, ts.lds[0], ts.lds[0] * ts.lds[1]};
+    const int stride[3] = {1, ts.lds[0], ts.lds[0] * ts.lds[1]};
+    const int max_loads = std::max(1, max_load_elems / ts.slot_elems);
 
     // sources of every target
     std::vector<std::vector<int>> src(tg.size());
     std::set<int> all_src;
     for (size_t t = 0; t < tg.size(); ++t) {
         std::set<int> s;
+        int nt = 0;
         for (int d = 0; d < 3; ++d) {
             const Csr &M = g.M[d];
             if (M.rp.empty()) continue;
+            nt += M.rp[tg[t] + 1] - M.rp[tg[t]];
             for (int p = M.rp[tg[t]]; p < M.rp[tg[t] + 1]; ++p) s.insert(M.ci[p]);
         }
+        if (nt > MAXT) return "a coupling row has more than 12 nonzeros";
+        out.max_terms = std::max(out.max_terms, nt);
         src[t].assign(s.begin(), s.end());
         all_src.insert(s.begin(), s.end());
         if ((int)s.size() > slot_capacity) return "a target couples to more components than fit in LDS";
@@ -157,8 +161,7 @@ std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slo
     };
     int used = 0;
     for (size_t q = 0; q < groups.size(); ++q) {
-        DevGroup gr;
-        gr.l0 = (int)out.loads.size();
+        std::vector<DevLoad> new_loads;
         for (int k : gsrc[q]) {
             if (slot_of[k] >= 0) continue;
             int s = -1;
@@ -192,20 +195,35 @@ std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slo
                     ld.ext[e] = 1;
                 }
             }
-            out.loads.push_back(ld);
+            new_loads.push_back(ld);
         }
+        // staging is bounded by the prefetch registers: leading load-only
+        // groups take what does not fit
+        size_t first = 0;
+        while (new_loads.size() - first > (size_t)max_loads) {
+            DevGroup lg;
+            lg.l0 = (int)out.loads.size();
+            out.loads.insert(out.loads.end(), new_loads.begin() + first,
+                             new_loads.begin() + first + max_loads);
+            lg.l1 = (int)out.loads.size();
+            lg.t0 = lg.t1 = (int)out.targets.size();
+            out.groups.push_back(lg);
+            first += max_loads;
+        }
+        DevGroup gr;
+        gr.l0 = (int)out.loads.size();
+        out.loads.insert(out.loads.end(), new_loads.begin() + first, new_loads.end());
         gr.l1 = (int)out.loads.size();
         gr.t0 = (int)out.targets.size();
         for (int t = groups[q].first; t < groups[q].second; ++t) {
             int j = tg[t];
-            DevTarget dt;
-            dt.comp = j;
-            dt.napp = kind == 0 ? 1 : 2;
-            dt.pad = 0;
-            for (int e = 0; e < 3; ++e) dt.ext[e] = e < g.ndim ? g.ext_of(j, g.axis[e]) : 1;
-            dt.q0 = (int)out.terms.size();
+            TgtRec r;
+            std::memset(&r, 0, sizeof(r));
+            r.comp = j;
+            for (int e = 0; e < 3; ++e) r.ext[e] = e < g.ndim ? g.ext_of(j, g.axis[e]) : 1;
             // x-terms, then y-terms, then z-terms, ascending k within each:
             // the summation order of sumcell (starmap_solver.m:350-351, :516-519)
+            int nt = 0;
             for (int d = 0; d < 3; ++d) {
                 const Csr &M = g.M[d];
                 if (M.rp.empty()) continue;
@@ -213,18 +231,17 @@ std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slo
                 for (int p = M.rp[j]; p < M.rp[j + 1]; ++p) {
                     int k = M.ci[p];
                     if (kd < 0) return "coupling along an axis without cells";
-                    DevTerm tm;
                     int off = slot_of[k] * ts.slot_elems;
                     for (int e = 0; e < g.ndim; ++e)
                         if (e != kd && g.gt[3 * k + g.axis[e]] == 1) off += stride[e];
-                    tm.off = off;
-                    tm.stride = stride[kd];
-                    tm.coef = -M.cv[p] / g.h[d];
-                    out.terms.push_back(tm);
+                    r.off[nt] = off;
+                    r.stride[nt] = stride[kd];
+                    r.coef[nt] = -M.cv[p] / g.h[d];
+                    ++nt;
                 }
             }
-            dt.q1 = (int)out.terms.size();
-            out.targets.push_back(dt);
+            r.nterm = nt;
+            out.targets.push_back(r);
         }
         gr.t1 = (int)out.targets.size();
         out.groups.push_back(gr);

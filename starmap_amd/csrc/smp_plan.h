@@ -4,9 +4,9 @@
 // difference quotients of the coupled components of the other parity
 // ("sources"), starmap_solver.m:313-359 (odd targets) and :361-414 (even
 // targets).  The device kernel processes one spatial tile per workgroup and
-// walks a schedule of groups: each group first stages the source tiles it
-// needs into LDS slots (tile + one-cell halo), then updates a few targets
-// from LDS.  This file builds that schedule once per problem.
+// walks a schedule of groups: while it updates the targets of group g from
+// LDS, it already fetches the source tiles and target values of group g+1.
+// This file builds that schedule once per problem.
 #pragma once
 
 #include <cstdint>
@@ -14,6 +14,8 @@
 #include <vector>
 
 namespace smp {
+
+constexpr int MAXT = 12;      // coupling terms per target record
 
 // device-visible records (POD, shared with the kernels)
 struct DevGroup {
@@ -28,18 +30,25 @@ struct DevLoad {
     int32_t ext[3];     // source extent per kernel axis (for clamping)
 };
 
-struct DevTerm {
-    int32_t off;        // LDS element offset of the low sample (slot base + shifts)
-    int32_t stride;     // LDS stride of the coupled axis (high = low + stride)
-    double coef;        // -M(j,k)/h
-};
-
-struct DevTarget {
+// One target of a substep: its coupling terms and its coefficients.  The
+// coefficient part is filled in by the runtime from the current material,
+// source and filter data.
+struct alignas(16) TgtRec {
     int32_t comp;
-    int32_t q0, q1;     // terms [q0,q1)
-    int32_t napp;       // 1 (odd substeps) or 2 (even substep: two half-steps)
+    int32_t nterm;
+    int32_t filt;       // apply the substep filter after each application
+    int32_t pad0;
     int32_t ext[3];     // target extent per kernel axis
-    int32_t pad;
+    int32_t pad1;
+    const void *sig;    // decay rate field (or null: sig_s)
+    const void *E;      // expm1div(-sig*dt/2) field (or null: E_s)
+    const void *Q;      // source field (or null: 0)
+    const void *F[2];   // filter fields of slots k=1,2 (or null: F_s)
+    double sig_s, E_s;
+    double F_s[2];
+    int32_t off[MAXT];     // LDS element offset of the low sample
+    int32_t stride[MAXT];  // LDS stride of the coupled axis
+    double coef[MAXT];     // -M(j,k)/h
 };
 
 struct Csr {
@@ -75,21 +84,24 @@ struct TileShape {
     int threads = 256;
 };
 
+// tile of tx*ty*tz cells (256 threads, one cell each)
+TileShape make_tile(const Geometry &g, int tx, int ty, int tz);
 TileShape choose_tile(const Geometry &g, int elem_bytes);
 
 struct Plan {
     std::vector<DevGroup> groups;
     std::vector<DevLoad> loads;
-    std::vector<DevTarget> targets;
-    std::vector<DevTerm> terms;
+    std::vector<TgtRec> targets;
     int n_slots = 0;
     int n_sources = 0;
+    int max_terms = 0;
 };
 
 // Build the schedule of one substep kind: targets = odd (kind 0) or even
-// (kind 1) components.  slot_capacity bounds the LDS slots; group_size is
-// the number of targets per group.
+// (kind 1) components.  slot_capacity bounds the LDS slots, group_size the
+// targets per group and max_load_elems the source elements a group may
+// stage (the kernel's prefetch registers).
 std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slot_capacity,
-                       int group_size, Plan &out);
+                       int group_size, int max_load_elems, Plan &out);
 
 }  // namespace smp

@@ -188,11 +188,14 @@ class HipSolver:
         _lib.check(_lib.lib().smp_step(self.handle, float(dt), int(n_steps)))
 
     def timing(self):
-        tot, sub = ctypes.c_double(), ctypes.c_double()
+        """(total ms, substep-kernel ms, substep launches, their algorithmic
+        bytes) of the last step() call, from HIP events on the handle's stream."""
+        tot, sub, by = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         nl = ctypes.c_int64()
         _lib.check(_lib.lib().smp_last_step_timing(self.handle, ctypes.byref(tot),
-                                                   ctypes.byref(sub), ctypes.byref(nl)))
-        return tot.value, sub.value, nl.value
+                                                   ctypes.byref(sub), ctypes.byref(nl),
+                                                   ctypes.byref(by)))
+        return tot.value, sub.value, nl.value, by.value
 
     def observables(self):
         s = np.zeros(self.n_sys)
@@ -281,6 +284,29 @@ class HipSolver:
                 v2 = capargs(par["filterfunction"], par, dt, s[3], *self.grid(g), t)
                 self.set_coeff(_lib.SMP_COEF_FILTER, g, s[3], 1, v1, shp)
                 self.set_coeff(_lib.SMP_COEF_FILTER, g, s[3], 2, v2, shp)
+
+    def autonomous(self):
+        par = self.par
+        return not (any(self.time_dependent)
+                    or (len(par["source_ind"]) > 0 and nargin(par["source"]) > 3)
+                    or (self.flag_filter and nargin(par["filterfunction"]) > 6))
+
+    def prepare(self):
+        """Host work of the first time step of an autonomous problem -- the
+        initial state, materials, time step, source and filter factors
+        (starmap_solver.m:140-146, :168-305 at t=0) -- after which every
+        step is ``step(dt, n)``.  Returns dt."""
+        if not self.autonomous():
+            raise ValueError("prepare() needs a problem without time dependence")
+        par = self.par
+        self.initial_state()
+        sA, sT = self.evaluate_materials(0.0, [True, True, self.flag_ani])
+        dt = min(self.dt_hyp, par["tfinal"] / par["cn"]) * par["cn"]
+        self.upload_materials(sA[(1, 1, 1)], sT)
+        self.upload_source(dt - dt / 2)
+        if self.flag_filter:
+            self.upload_filter(dt, dt)
+        return dt
 
     # -- the time loop ----------------------------------------------------
     def run(self, verbose=True):

@@ -74,3 +74,39 @@ def test_init_errors_match_reference_messages():
     with pytest.raises(ValueError, match="sigma_a has incompatible number of arguments"):
         I.starmap_init({"closure": "P", "n_mom": 1, "ax": [0, 1, 0, 1],
                         "sigma_a": lambda x: 0.0})
+
+
+@pytest.mark.parametrize("clo,N,dim", [("P", 5, 2), ("P", 39, 2), ("P", 7, 3), ("SP", 9, 2),
+                                       ("SP", 7, 3), ("P", 4, 1)])
+def test_host_only_handle_grid_types_and_schedule(clo, N, dim):
+    """A handle created with device = -1 validates the coupling, computes the
+    grid types (equal to the reference's class closure, restated by the
+    oracle) and builds the substep schedules without a GPU."""
+    from starmap_amd.solver import HipSolver
+    ax = [0.0, 1.0] * dim
+    prob = {"closure": clo, "n_mom": N, "ax": ax, "n": [12] * dim, "bc": [1] * dim}
+    par = I.starmap_init(prob)
+    s = HipSolver(par, device=-1)
+    gt_ref, _ = so.grid_classes(par["Mx"], par["My"], par["Mz"])
+    assert np.array_equal(s.gt, gt_ref)
+    for kind in (0, 1):
+        groups, slots, loads, sources = s.plan_stats(kind)
+        assert groups >= 1 and slots >= 1
+        # every source staged once per tile when the LDS holds the live set
+        assert loads == sources
+    from starmap_amd._lib import SmpError
+    with pytest.raises(SmpError, match="without a device"):
+        s.step(0.1, 1)
+    s.close()
+
+
+def test_invalid_coupling_is_rejected():
+    from starmap_amd.solver import HipSolver
+    from starmap_amd._lib import SmpError
+    par = I.starmap_init({"closure": "P", "n_mom": 3, "ax": [0, 1, 0, 1], "n": [8, 8]})
+    bad = dict(par)
+    My = par["My"].copy()
+    My[0, 0] = 1.0             # a self-coupling breaks the two-colouring
+    bad["My"] = My
+    with pytest.raises(SmpError, match="not bipartite"):
+        HipSolver(bad, device=-1)

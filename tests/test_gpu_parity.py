@@ -145,3 +145,23 @@ def test_errors_are_reported_not_crashes():
     with pytest.raises(SmpError, match="dt must be positive"):
         s.step(-1.0, 1)
     s.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fused_steps_bitwise_equal_single_steps(precision):
+    """n steps in one call (substep 3 fused with the next substep 1) give the
+    same bits as n calls of one step (three launches each)."""
+    prob = P.lattice_2d(n=45, n_mom=9, tfinal=1.0)
+    outs = []
+    for split in (False, True):
+        s = HipSolver(I.starmap_init(prob), precision=precision)
+        dt = s.prepare()
+        if split:
+            for _ in range(7):
+                s.step(dt, 1)
+        else:
+            s.step(dt, 7)
+        outs.append([s.get_state(j) for j in range(s.n_sys)])
+        s.close()
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
