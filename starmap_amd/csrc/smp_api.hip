@@ -13,7 +13,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <cstdlib>
 #include <map>
+#include <set>
 #include <memory>
 #include <string>
 #include <tuple>
@@ -80,6 +82,10 @@ struct smp_problem {
     void *plan_dev[2] = {nullptr, nullptr};
     size_t plan_off[2][3] = {{0, 0, 0}, {0, 0, 0}};   // loads, targets offsets
     size_t lds_bytes[2] = {0, 0};
+    int ncf_cap[2] = {0, 0};       // coefficient fields staged per tile (LDS reserved)
+    int ncf[2] = {0, 0};           // coefficient fields currently staged
+    void *cf_dev[2] = {nullptr, nullptr};
+    int blocks_per_cu = 0;
     bool host_only = false;
     std::vector<int32_t> mom_order, f_order;
     Field sA1, EA;
@@ -276,17 +282,40 @@ int smp_create(const smp_desc *d, smp_problem **out) {
     h->slab = (int64_t)h->pitch0 * h->E[1] * h->E[2];
     h->slab = ((h->slab + 63) / 64) * 64;
     h->ts = smp::choose_tile(h->geo, h->elem);
+    if (h->ts.lds[0] > 1024 || h->ts.lds[1] > 128 || h->ts.lds[2] > 128 || h->ts.threads != 256)
+        return fail(SMP_E_INVALID, "unsupported tile shape");
     h->n_tiles = 1;
     for (int e = 0; e < 3; ++e) {
         h->ntile[e] = (h->E[e] + h->ts.tile[e] - 1) / h->ts.tile[e];
         h->n_tiles *= h->ntile[e];
     }
     // schedules
-    const size_t lds_budget = 160 * 1024 - smp::substep_lds_overhead();
-    const int cap = (int)(lds_budget / ((size_t)h->ts.slot_elems * h->elem));
+    // Workgroups per CU to aim for (SMP_BLOCKS_PER_CU overrides; default 2,
+    // the most the substep kernel's registers allow): fewer LDS slots per
+    // workgroup trade source re-staging (mostly L2 / Infinity Cache hits)
+    // for occupancy.  The schedule still shrinks to the smallest slot count
+    // that stages every source once when that fits.
+    int want_blocks = 2;
+    if (const char *env = std::getenv("SMP_BLOCKS_PER_CU")) want_blocks = std::atoi(env);
+    if (want_blocks < 0 || want_blocks > 8) return fail(SMP_E_INVALID, "SMP_BLOCKS_PER_CU must be 0..8");
+    h->blocks_per_cu = want_blocks;
     for (int kind = 0; kind < 2; ++kind) {
+        // coefficient fields staged per tile: decay rate and factor of each
+        // (grid type, order) among the targets, plus one source field
+        std::set<std::tuple<int, int, int, int>> keys;
+        const std::vector<int32_t> &tg = kind == 0 ? h->geo.odd : h->geo.even;
+        for (int j : tg)
+            keys.insert(std::make_tuple(h->geo.gt[3 * j], h->geo.gt[3 * j + 1], h->geo.gt[3 * j + 2],
+                                        j == 0 ? -1 : d->mom_order[j]));
+        h->ncf_cap[kind] = std::min<int>(smp::MAX_CFIELDS, 2 * (int)keys.size() + 1);
+        const size_t fixed = smp::substep_lds_overhead() +
+                             (size_t)h->ncf_cap[kind] * h->ts.threads * h->elem;
+        const size_t per_block = want_blocks > 0 ? 160 * 1024 / want_blocks : 160 * 1024;
+        if (per_block <= fixed + (size_t)h->ts.slot_elems * h->elem)
+            return fail(SMP_E_INVALID, "LDS too small for the requested workgroups per CU");
+        const int cap = (int)((per_block - fixed) / ((size_t)h->ts.slot_elems * h->elem));
         err = smp::build_plan(h->geo, h->ts, kind, cap, smp::GMAX, smp::PF * h->ts.threads,
-                              h->plan[kind]);
+                              smp::MAXL, h->plan[kind]);
         if (!err.empty()) return fail(SMP_E_INVALID, err);
         // the smallest slot count that still stages every source once per
         // tile: less LDS per workgroup, more workgroups per CU
@@ -296,21 +325,21 @@ int smp_create(const smp_desc *d, smp_problem **out) {
                 const int mid = (lo + hi) / 2;
                 smp::Plan trial;
                 std::string e2 = smp::build_plan(h->geo, h->ts, kind, mid, smp::GMAX,
-                                                 smp::PF * h->ts.threads, trial);
+                                                 smp::PF * h->ts.threads, smp::MAXL, trial);
                 if (e2.empty() && (int)trial.loads.size() == trial.n_sources)
                     hi = mid;
                 else
                     lo = mid + 1;
             }
             err = smp::build_plan(h->geo, h->ts, kind, lo, smp::GMAX, smp::PF * h->ts.threads,
-                                  h->plan[kind]);
+                                  smp::MAXL, h->plan[kind]);
             if (!err.empty()) return fail(SMP_E_INVALID, err);
         }
+        h->lds_bytes[kind] = fixed + (size_t)std::max(h->plan[kind].n_slots, 1) *
+                                         h->ts.slot_elems * h->elem;
         const smp::Plan &p = h->plan[kind];
         if ((int)p.groups.size() > smp::MAX_GROUPS)
             return fail(SMP_E_INVALID, "substep schedule has too many groups");
-        h->lds_bytes[kind] = smp::substep_lds_overhead() +
-                             (size_t)std::max(p.n_slots, 1) * h->ts.slot_elems * h->elem;
         auto al = [](size_t v) { return (v + 255) / 256 * 256; };
         size_t s0 = p.groups.size() * sizeof(smp::DevGroup);
         size_t s1 = p.loads.size() * sizeof(smp::DevLoad);
@@ -514,15 +543,18 @@ int refresh_decay(smp_problem *h, double dt) {
 template <typename T>
 int refresh_coefs(smp_problem *h) {
     const int n = h->geo.n_sys;
-    // per-component coefficient part of the target records
-    std::vector<smp::TgtRec> cf(n);
+    struct CF {
+        const Field *sig, *E, *Q;
+        int filt;
+        const void *F[2];
+        double F_s[2];
+    };
+    std::vector<CF> cf(n);
     for (int j = 0; j < n; ++j) {
-        smp::TgtRec &c = cf[j];
-        std::memset(&c, 0, sizeof(c));
-        const Field *sig, *E;
+        CF &c = cf[j];
         if (j == 0) {
-            sig = &h->sA1;
-            E = &h->EA;
+            c.sig = &h->sA1;
+            c.E = &h->EA;
         } else {
             Key k(h->geo.gt[3 * j], h->geo.gt[3 * j + 1], h->geo.gt[3 * j + 2], h->mom_order[j], 0);
             auto it = h->sT.find(k);
@@ -530,14 +562,11 @@ int refresh_coefs(smp_problem *h) {
                 return fail(SMP_E_STATE, "sigma_t not set for the grid type and order of component " +
                                              std::to_string(j));
             }
-            sig = &it->second;
-            E = &h->ET[k];
+            c.sig = &it->second;
+            c.E = &h->ET[k];
         }
-        c.sig = sig->dev;
-        c.sig_s = sig->scalar;
-        c.E = E->dev;
-        c.E_s = E->scalar;
-        c.Q = h->Q[j].set ? h->Q[j].dev : nullptr;
+        c.Q = h->Q[j].set ? &h->Q[j] : nullptr;
+        c.F[0] = c.F[1] = nullptr;
         c.F_s[0] = c.F_s[1] = 1.0;
         c.filt = 0;
         if (j > 0 && h->f_position == SMP_FILTER_SUBSTEP) {
@@ -558,20 +587,55 @@ int refresh_coefs(smp_problem *h) {
         }
     }
     for (int kind = 0; kind < 2; ++kind) {
+        // coefficient fields of this kind's targets, staged per tile in LDS
+        // up to the reserved count, read from global memory beyond it
+        std::vector<const void *> table;
+        auto place = [&](const Field *f, int32_t &off, const void *&p, double &sc) {
+            p = nullptr;
+            sc = 0.0;
+            if (!f) {
+                off = -1;
+                return;
+            }
+            if (!f->dev) {
+                off = -1;
+                sc = f->scalar;
+                return;
+            }
+            for (size_t c = 0; c < table.size(); ++c)
+                if (table[c] == f->dev) {
+                    off = (int32_t)(c * h->ts.threads);
+                    return;
+                }
+            if ((int)table.size() < h->ncf_cap[kind]) {
+                off = (int32_t)(table.size() * h->ts.threads);
+                table.push_back(f->dev);
+                return;
+            }
+            off = -2;
+            p = f->dev;
+        };
         std::vector<smp::TgtRec> recs = h->plan[kind].targets;
         for (smp::TgtRec &r : recs) {
-            const smp::TgtRec &c = cf[r.comp];
+            const CF &c = cf[r.comp];
+            double qs = 0.0;
+            place(c.sig, r.sig_off, r.sig_p, r.sig_s);
+            place(c.E, r.E_off, r.E_p, r.E_s);
+            place(c.Q, r.Q_off, r.Q_p, qs);
             r.filt = c.filt;
-            r.sig = c.sig;
-            r.E = c.E;
-            r.Q = c.Q;
             r.F[0] = c.F[0];
             r.F[1] = c.F[1];
-            r.sig_s = c.sig_s;
-            r.E_s = c.E_s;
             r.F_s[0] = c.F_s[0];
             r.F_s[1] = c.F_s[1];
         }
+        h->ncf[kind] = (int)table.size();
+        if (!h->cf_dev[kind]) {
+            int rc = h->dalloc(&h->cf_dev[kind], sizeof(void *) * smp::MAX_CFIELDS);
+            if (rc) return rc;
+        }
+        if (!table.empty())
+            HIP_TRY(hipMemcpy(h->cf_dev[kind], table.data(), sizeof(void *) * table.size(),
+                              hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy((char *)h->plan_dev[kind] + h->plan_off[kind][1], recs.data(),
                           recs.size() * sizeof(smp::TgtRec), hipMemcpyHostToDevice));
     }
@@ -607,6 +671,9 @@ smp::SubstepArgs<T> make_args(smp_problem *h, int kind, double dt, int napp, int
     a.loads = (const smp::DevLoad *)(base + h->plan_off[kind][0]);
     a.targets = (const smp::TgtRec *)(base + h->plan_off[kind][1]);
     a.ngroups = (int)p.groups.size();
+    a.cfields = (const void *const *)h->cf_dev[kind];
+    a.ncf = h->ncf[kind];
+    for (int e = 0; e < 3; ++e) a.E[e] = h->E[e];
     a.half_dt = dt / 2;
     a.napp = napp;
     a.fslot[0] = fs0;
@@ -670,7 +737,8 @@ int run_steps(smp_problem *h, double dt, int64_t n_steps) {
         }
         const int kind = p == &even ? 1 : 0;
         if (li < timed) HIP_TRY(hipEventRecord(h->events[2 + 2 * li], h->stream));
-        HIP_TRY(smp::launch_substep<T>(*p, h->ts.threads, h->lds_bytes[kind], h->stream));
+        HIP_TRY(smp::launch_substep<T>(*p, h->plan[kind].max_terms, h->ts.threads,
+                                       h->lds_bytes[kind], h->stream));
         if (li < timed) HIP_TRY(hipEventRecord(h->events[3 + 2 * li], h->stream));
         ++li;
     }

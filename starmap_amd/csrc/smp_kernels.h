@@ -9,7 +9,9 @@ namespace smp {
 
 constexpr int GMAX = 8;       // targets per group
 constexpr int PF = 16;        // staged source elements per thread per group
+constexpr int MAXL = 16;      // source loads per group
 constexpr int MAX_GROUPS = 256;
+constexpr int MAX_CFIELDS = 12; // coefficient fields staged per tile
 
 template <typename T>
 struct SubstepArgs {
@@ -24,16 +26,21 @@ struct SubstepArgs {
     const DevGroup *groups;
     const DevLoad *loads;
     const TgtRec *targets;
+    const void *const *cfields;   // coefficient fields staged per tile
+    int32_t ncf;
+    int32_t E[3];             // storage extents per kernel axis
     double half_dt;
     int32_t napp;             // half-step applications with the same flux (1 or 2)
     int32_t fslot[2];         // filter slot of application 0 and 1
 };
 
-// LDS bytes the kernel needs besides the source slots
+// LDS bytes the kernel needs besides the source slots and the staged
+// coefficient fields (ncf * threads elements)
 size_t substep_lds_overhead();
 
+// nt: terms per target the kernel unrolls (4, 8 or 12)
 template <typename T>
-hipError_t launch_substep(const SubstepArgs<T> &a, int threads, size_t lds_bytes,
+hipError_t launch_substep(const SubstepArgs<T> &a, int nt, int threads, size_t lds_bytes,
                           hipStream_t s);
 
 template <typename T>

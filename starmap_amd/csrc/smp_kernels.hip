@@ -40,26 +40,34 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 
 }  // namespace
 
-size_t substep_lds_overhead() {
-    return 2 * GMAX * sizeof(TgtRec) + MAX_GROUPS * sizeof(DevGroup);
-}
+// LDS layout: group headers | 2 x (GMAX target records, MAXL load records) |
+// source slots
+constexpr int BLOB_REC = GMAX * (int)sizeof(TgtRec);
+constexpr int BLOB = BLOB_REC + MAXL * (int)sizeof(DevLoad);
 
-// One workgroup = one spatial tile, one thread = one cell.  LDS holds the
-// group headers, two buffers of target records and the source slots.  While
-// group g is updated from LDS, the source tiles and target values of group
-// g+1 and the records of group g+2 are in flight in registers.
-template <typename T>
+size_t substep_lds_overhead() { return MAX_GROUPS * sizeof(DevGroup) + 2 * BLOB; }
+
+// One workgroup = one spatial tile, one thread = one cell.  While the targets
+// of group g are updated from LDS and registers, the source tiles and target
+// values of group g+1 and the records of group g+2 are in flight; the update
+// itself waits on no global load, so the in-order load counter never makes
+// it wait for the prefetch.  The coefficient fields the targets use (decay
+// rates, decay factors, sources) are staged per tile once.
+template <typename T, int NT>
 __global__ __launch_bounds__(256) void substep_kernel(const SubstepArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    TgtRec *rec = reinterpret_cast<TgtRec *>(smem_raw);
-    DevGroup *grp = reinterpret_cast<DevGroup *>(smem_raw + 2 * GMAX * sizeof(TgtRec));
-    T *lds = reinterpret_cast<T *>(smem_raw + 2 * GMAX * sizeof(TgtRec) +
-                                   MAX_GROUPS * sizeof(DevGroup));
+    DevGroup *grp = reinterpret_cast<DevGroup *>(smem_raw);
+    unsigned char *blob = smem_raw + MAX_GROUPS * sizeof(DevGroup);
+    T *cfa = reinterpret_cast<T *>(blob + 2 * BLOB);
+    const int nthr = blockDim.x;
+    T *lds = cfa + a.ncf * nthr;
     constexpr int RECV = sizeof(TgtRec) / 16;
+    constexpr int LDV = sizeof(DevLoad) / 16;   // DevLoad is 32 bytes
+    static_assert(sizeof(DevLoad) == 32, "DevLoad layout");
+    static_assert(GMAX * RECV + MAXL * LDV <= 256, "one blob chunk per thread");
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= a.n_tiles) return;
     const int tid = threadIdx.x;
-    const int nthr = blockDim.x;
     int b = tile;
     const int t0 = b % a.ntile[0];
     b /= a.ntile[0];
@@ -70,6 +78,7 @@ __global__ __launch_bounds__(256) void substep_kernel(const SubstepArgs<T> a) {
     const int c1 = (tid / a.tile[0]) % a.tile[1];
     const int c2 = tid / (a.tile[0] * a.tile[1]);
     const int g0 = o0 + c0, g1 = o1 + c1, g2 = o2 + c2;
+    const bool in_store = g0 < a.E[0] && g1 < a.E[1] && g2 < a.E[2];
     const int lbase = c0 + a.lds[0] * (c1 + a.lds[1] * c2);
     const int64_t gidx = g0 + (int64_t)a.pitch0 * (g1 + (int64_t)a.ext1 * g2);
     const int S = a.slot_elems;
@@ -78,33 +87,50 @@ __global__ __launch_bounds__(256) void substep_kernel(const SubstepArgs<T> a) {
     const int ng = a.ngroups;
 
     for (int i = tid; i < ng; i += nthr) grp[i] = a.groups[i];
+    for (int c = 0; c < a.ncf; ++c)
+        cfa[c * nthr + tid] = in_store ? reinterpret_cast<const T *>(a.cfields[c])[gidx] : (T)0;
+
+    // the staged elements of this thread: element e = tid + k*nthr of a
+    // group's load list is slot-local element r of load li; its tile-local
+    // coordinates are fixed for the whole kernel
+    int epack[PF];
+#pragma unroll
+    for (int k = 0; k < PF; ++k) {
+        const int e = tid + k * nthr;
+        const int li = e / S;
+        const int r = e - li * S;
+        const int l0 = r % L0;
+        const int rr = r / L0;
+        // l0 < 1024, l1 < 128, l2 < 128, li < 128 (checked by the host)
+        epack[k] = l0 | ((rr % L1) << 10) | ((rr / L1) << 17) | (li << 24);
+    }
 
     T pf[PF];
     int pdst[PF];
-    int4 rpf = make_int4(0, 0, 0, 0);
-    T un_cur[GMAX], un_nxt[GMAX];
+    int4 bpf = make_int4(0, 0, 0, 0);
+    T u_cur[GMAX], u_nxt[GMAX];
 
-    // global -> registers: the source tiles of group gq
-    auto fetch_src = [&](const DevGroup &gr) {
-        const int total = (gr.l1 - gr.l0) * S;
+    auto recs = [&](int buf) { return reinterpret_cast<TgtRec *>(blob + buf * BLOB); };
+    auto ldr = [&](int buf) { return reinterpret_cast<DevLoad *>(blob + buf * BLOB + BLOB_REC); };
+
+    // global -> registers: the source tiles of a group (load records in LDS)
+    auto fetch_src = [&](const DevGroup &gr, int buf) {
+        const int nl = gr.l1 - gr.l0;
+        const DevLoad *L = ldr(buf);
 #pragma unroll
         for (int k = 0; k < PF; ++k) {
-            const int e = tid + k * nthr;
+            const int li = epack[k] >> 24;
             pdst[k] = -1;
-            if (e < total) {
-                const int li = e / S;
-                const int r = e - li * S;
-                const DevLoad ld = a.loads[gr.l0 + li];
-                const int l0 = r % L0;
-                const int rr = r / L0;
-                const int l1 = rr % L1;
-                const int l2 = rr / L1;
+            if (li < nl) {
+                const DevLoad &ld = L[li];
+                const int l0 = epack[k] & 1023, l1 = (epack[k] >> 10) & 127,
+                          l2 = (epack[k] >> 17) & 127;
                 const int x0 = wrap_or_clamp(o0 + l0 + ld.org[0], a.n[0], a.bc[0], ld.ext[0]);
                 const int x1 = wrap_or_clamp(o1 + l1 + ld.org[1], a.n[1], a.bc[1], ld.ext[1]);
                 const int x2 = wrap_or_clamp(o2 + l2 + ld.org[2], a.n[2], a.bc[2], ld.ext[2]);
                 pf[k] = a.U[(int64_t)ld.comp * a.slab + x0 +
                             (int64_t)a.pitch0 * (x1 + (int64_t)a.ext1 * x2)];
-                pdst[k] = ld.slot * S + r;
+                pdst[k] = ld.slot * S + l0 + L0 * (l1 + L1 * l2);
             }
         }
     };
@@ -113,47 +139,66 @@ __global__ __launch_bounds__(256) void substep_kernel(const SubstepArgs<T> a) {
         for (int k = 0; k < PF; ++k)
             if (pdst[k] >= 0) lds[pdst[k]] = pf[k];
     };
-    // global -> registers -> LDS: the target records of a group
-    auto fetch_rec = [&](const DevGroup &gr) {
-        if (tid < (gr.t1 - gr.t0) * RECV)
-            rpf = reinterpret_cast<const int4 *>(a.targets + gr.t0)[tid];
+    // global -> registers -> LDS: target and load records of a group
+    auto fetch_blob = [&](const DevGroup &gr) {
+        const int nr = (gr.t1 - gr.t0) * RECV;
+        const int nl = (gr.l1 - gr.l0) * LDV;
+        if (tid < nr)
+            bpf = reinterpret_cast<const int4 *>(a.targets + gr.t0)[tid];
+        else if (tid >= GMAX * RECV && tid < GMAX * RECV + nl)
+            bpf = reinterpret_cast<const int4 *>(a.loads + gr.l0)[tid - GMAX * RECV];
     };
-    auto commit_rec = [&](const DevGroup &gr, int buf) {
-        if (tid < (gr.t1 - gr.t0) * RECV)
-            reinterpret_cast<int4 *>(rec + buf * GMAX)[tid] = rpf;
+    auto commit_blob = [&](const DevGroup &gr, int buf) {
+        const int nr = (gr.t1 - gr.t0) * RECV;
+        const int nl = (gr.l1 - gr.l0) * LDV;
+        if (tid < nr || (tid >= GMAX * RECV && tid < GMAX * RECV + nl))
+            reinterpret_cast<int4 *>(blob + buf * BLOB)[tid] = bpf;
     };
-    auto load_targets = [&](const DevGroup &gr, int buf, T *dst) {
-        const TgtRec *r = rec + buf * GMAX;
+    // global -> registers: the target values of a group
+    auto load_targets = [&](const DevGroup &gr, int buf, T *u) {
+        const TgtRec *r = recs(buf);
+        const int nt = gr.t1 - gr.t0;
 #pragma unroll
         for (int i = 0; i < GMAX; ++i) {
-            if (i < gr.t1 - gr.t0 && g0 < r[i].ext[0] && g1 < r[i].ext[1] && g2 < r[i].ext[2])
-                dst[i] = a.U[(int64_t)r[i].comp * a.slab + gidx];
+            if (i < nt) {
+                const TgtRec &ri = r[i];
+                if (g0 < ri.ext[0] && g1 < ri.ext[1] && g2 < ri.ext[2])
+                    u[i] = a.U[(int64_t)ri.comp * a.slab + gidx];
+            }
         }
+    };
+    auto coef = [&](int off, const void *p, double s) -> T {
+        if (off >= 0) return cfa[off + tid];
+        if (off == -1) return (T)s;
+        return reinterpret_cast<const T *>(p)[gidx];
     };
 
     __syncthreads();
     if (ng > 0) {
-        fetch_src(grp[0]);
-        fetch_rec(grp[0]);
-        commit_src();
-        commit_rec(grp[0], 0);
+        fetch_blob(grp[0]);
+        commit_blob(grp[0], 0);
         if (ng > 1) {
-            fetch_rec(grp[1]);
-            commit_rec(grp[1], 1);
+            fetch_blob(grp[1]);
+            commit_blob(grp[1], 1);
         }
     }
     __syncthreads();
-    if (ng > 0) load_targets(grp[0], 0, un_cur);
+    if (ng > 0) {
+        fetch_src(grp[0], 0);
+        commit_src();
+        load_targets(grp[0], 0, u_cur);
+    }
+    __syncthreads();
 
     for (int gq = 0; gq < ng; ++gq) {
         const DevGroup gr = grp[gq];
         const int buf = gq & 1;
         if (gq + 1 < ng) {
-            load_targets(grp[gq + 1], buf ^ 1, un_nxt);
-            fetch_src(grp[gq + 1]);
+            load_targets(grp[gq + 1], buf ^ 1, u_nxt);
+            fetch_src(grp[gq + 1], buf ^ 1);
         }
-        if (gq + 2 < ng) fetch_rec(grp[gq + 2]);
-        const TgtRec *rb = rec + buf * GMAX;
+        if (gq + 2 < ng) fetch_blob(grp[gq + 2]);
+        const TgtRec *rb = recs(buf);
         const int nt = gr.t1 - gr.t0;
 #pragma unroll
         for (int i = 0; i < GMAX; ++i) {
@@ -162,18 +207,18 @@ __global__ __launch_bounds__(256) void substep_kernel(const SubstepArgs<T> a) {
                 if (g0 < r.ext[0] && g1 < r.ext[1] && g2 < r.ext[2]) {
                     // W = -sumcell(...): the coefficients carry -M(j,k)/h
                     T w = 0;
-                    const int nterm = r.nterm;
+                    if (r.nterm > 0) {
 #pragma unroll
-                    for (int q = 0; q < MAXT; ++q) {
-                        if (q < nterm) {
-                            const int o = r.off[q] + lbase;
-                            w += (T)r.coef[q] * (lds[o + r.stride[q]] - lds[o]);
+                        for (int q = 0; q < NT; ++q) {
+                            const Term tm = r.t[q];
+                            const int o = tm.off + lbase;
+                            w += (T)tm.coef * (lds[o + tm.stride] - lds[o]);
                         }
                     }
-                    const T sig = r.sig ? reinterpret_cast<const T *>(r.sig)[gidx] : (T)r.sig_s;
-                    const T E = r.E ? reinterpret_cast<const T *>(r.E)[gidx] : (T)r.E_s;
-                    const T wq = r.Q ? w + reinterpret_cast<const T *>(r.Q)[gidx] : w;
-                    T u = un_cur[i];
+                    const T sig = coef(r.sig_off, r.sig_p, r.sig_s);
+                    const T E = coef(r.E_off, r.E_p, r.E_s);
+                    const T wq = r.Q_off == -1 ? w : w + coef(r.Q_off, r.Q_p, 0.0);
+                    T u = u_cur[i];
                     for (int app = 0; app < a.napp; ++app) {
                         // U + dt/2*(W+Q - s.*U).*E   (starmap_solver.m:352-354)
                         u = u + (half_dt * (wq - sig * u)) * E;
@@ -189,28 +234,36 @@ __global__ __launch_bounds__(256) void substep_kernel(const SubstepArgs<T> a) {
         }
         __syncthreads();
         if (gq + 1 < ng) commit_src();
-        if (gq + 2 < ng) commit_rec(grp[gq + 2], buf);
+        if (gq + 2 < ng) commit_blob(grp[gq + 2], buf);
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < GMAX; ++i) un_cur[i] = un_nxt[i];
+        for (int i = 0; i < GMAX; ++i) u_cur[i] = u_nxt[i];
     }
 }
 
-template <typename T>
-hipError_t launch_substep(const SubstepArgs<T> &a, int threads, size_t lds_bytes,
-                          hipStream_t s) {
+template <typename T, int NT>
+static hipError_t launch_nt(const SubstepArgs<T> &a, int threads, size_t lds_bytes,
+                            hipStream_t s) {
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void *)substep_kernel<T>,
+        hipError_t e = hipFuncSetAttribute((const void *)substep_kernel<T, NT>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            160 * 1024);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    if (a.ngroups > MAX_GROUPS) return hipErrorInvalidValue;
     const int nb = ((a.n_tiles + 7) / 8) * 8;
-    hipLaunchKernelGGL(substep_kernel<T>, dim3(nb), dim3(threads), lds_bytes, s, a);
+    hipLaunchKernelGGL((substep_kernel<T, NT>), dim3(nb), dim3(threads), lds_bytes, s, a);
     return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_substep(const SubstepArgs<T> &a, int nt, int threads, size_t lds_bytes,
+                          hipStream_t s) {
+    if (a.ngroups > MAX_GROUPS || a.ncf > MAX_CFIELDS) return hipErrorInvalidValue;
+    if (nt <= 4) return launch_nt<T, 4>(a, threads, lds_bytes, s);
+    if (nt <= 8) return launch_nt<T, 8>(a, threads, lds_bytes, s);
+    return launch_nt<T, MAXT>(a, threads, lds_bytes, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -359,7 +412,7 @@ hipError_t launch_reduce_final(const double *part, int blocks, double *out3, hip
 }
 
 #define SMP_INSTANTIATE(T)                                                                     \
-    template hipError_t launch_substep<T>(const SubstepArgs<T> &, int, size_t, hipStream_t);  \
+    template hipError_t launch_substep<T>(const SubstepArgs<T> &, int, int, size_t, hipStream_t); \
     template hipError_t launch_decay<T>(const T *, T *, int64_t, double, hipStream_t);        \
     template hipError_t launch_scale<T>(T *, const T *, double, int64_t, hipStream_t);        \
     template hipError_t launch_interp<T>(const T *, const T *, double *, int64_t, double,     \

@@ -103,13 +103,13 @@ TileShape choose_tile(const Geometry &g, int elem_bytes) {
 }
 
 std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slot_capacity,
-                       int group_size, int max_load_elems, Plan &out) {
+                       int group_size, int max_load_elems, int max_loads_per_group, Plan &out) {
     out = Plan();
     const std::vector<int32_t> &tg = kind == 0 ? g.odd : g.even;
     int kaxis_of[3] = {-1, -1, -1};
     for (int e = 0; e < g.ndim; ++e) kaxis_of[g.axis[e]] = e;
     const int stride[3] = {1, ts.lds[0], ts.lds[0] * ts.lds[1]};
-    const int max_loads = std::max(1, max_load_elems / ts.slot_elems);
+    const int max_loads = std::max(1, std::min(max_loads_per_group, max_load_elems / ts.slot_elems));
 
     // sources of every target
     std::vector<std::vector<int>> src(tg.size());
@@ -234,13 +234,20 @@ std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slo
                     int off = slot_of[k] * ts.slot_elems;
                     for (int e = 0; e < g.ndim; ++e)
                         if (e != kd && g.gt[3 * k + g.axis[e]] == 1) off += stride[e];
-                    r.off[nt] = off;
-                    r.stride[nt] = stride[kd];
-                    r.coef[nt] = -M.cv[p] / g.h[d];
+                    r.t[nt].off = off;
+                    r.t[nt].stride = stride[kd];
+                    r.t[nt].coef = -M.cv[p] / g.h[d];
                     ++nt;
                 }
             }
             r.nterm = nt;
+            // padding terms (coefficient 0, stride 0, reading the first real
+            // term's staged sample) keep the kernel's term loop free of branches
+            for (int q = nt; q < MAXT; ++q) {
+                r.t[q].off = nt > 0 ? r.t[0].off : 0;
+                r.t[q].stride = 0;
+                r.t[q].coef = 0.0;
+            }
             out.targets.push_back(r);
         }
         gr.t1 = (int)out.targets.size();

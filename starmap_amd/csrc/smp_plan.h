@@ -30,25 +30,30 @@ struct DevLoad {
     int32_t ext[3];     // source extent per kernel axis (for clamping)
 };
 
+// One coupling term: W += coef * (lds[off + lbase + stride] - lds[off + lbase])
+struct alignas(16) Term {
+    int32_t off;        // LDS element offset of the low sample (slot base + shifts)
+    int32_t stride;     // LDS stride of the coupled axis
+    double coef;        // -M(j,k)/h; 0 for padding terms
+};
+
 // One target of a substep: its coupling terms and its coefficients.  The
 // coefficient part is filled in by the runtime from the current material,
-// source and filter data.
+// source and filter data.  A coefficient is a scalar (*_off == -1), a field
+// staged per tile in LDS (*_off >= 0: offset of its tile) or a field read
+// from global memory (*_off == -2, pointer *_p).
 struct alignas(16) TgtRec {
     int32_t comp;
-    int32_t nterm;
     int32_t filt;       // apply the substep filter after each application
-    int32_t pad0;
+    int32_t sig_off, E_off;
+    int32_t Q_off;
     int32_t ext[3];     // target extent per kernel axis
-    int32_t pad1;
-    const void *sig;    // decay rate field (or null: sig_s)
-    const void *E;      // expm1div(-sig*dt/2) field (or null: E_s)
-    const void *Q;      // source field (or null: 0)
+    const void *sig_p, *E_p, *Q_p;
+    int32_t nterm, pad;
     const void *F[2];   // filter fields of slots k=1,2 (or null: F_s)
     double sig_s, E_s;
     double F_s[2];
-    int32_t off[MAXT];     // LDS element offset of the low sample
-    int32_t stride[MAXT];  // LDS stride of the coupled axis
-    double coef[MAXT];     // -M(j,k)/h
+    Term t[MAXT];
 };
 
 struct Csr {
@@ -99,9 +104,9 @@ struct Plan {
 
 // Build the schedule of one substep kind: targets = odd (kind 0) or even
 // (kind 1) components.  slot_capacity bounds the LDS slots, group_size the
-// targets per group and max_load_elems the source elements a group may
-// stage (the kernel's prefetch registers).
+// targets per group, max_load_elems the source elements a group may stage
+// (the kernel's prefetch registers) and max_loads_per_group its load records.
 std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slot_capacity,
-                       int group_size, int max_load_elems, Plan &out);
+                       int group_size, int max_load_elems, int max_loads_per_group, Plan &out);
 
 }  // namespace smp
