@@ -78,7 +78,7 @@ def test_init_errors_match_reference_messages():
 
 @pytest.mark.parametrize("clo,N,dim", [("P", 5, 2), ("P", 39, 2), ("P", 7, 3), ("SP", 9, 2),
                                        ("SP", 7, 3), ("P", 4, 1)])
-def test_host_only_handle_grid_types_and_schedule(clo, N, dim):
+def test_host_only_handle_grid_types_and_schedule(clo, N, dim, monkeypatch):
     """A handle created with device = -1 validates the coupling, computes the
     grid types (equal to the reference's class closure, restated by the
     oracle) and builds the substep schedules without a GPU."""
@@ -86,14 +86,23 @@ def test_host_only_handle_grid_types_and_schedule(clo, N, dim):
     ax = [0.0, 1.0] * dim
     prob = {"closure": clo, "n_mom": N, "ax": ax, "n": [12] * dim, "bc": [1] * dim}
     par = I.starmap_init(prob)
+    # one workgroup per CU: the whole LDS holds the live set, every source is
+    # staged once per tile
+    monkeypatch.setenv("SMP_BLOCKS_PER_CU", "1")
     s = HipSolver(par, device=-1)
     gt_ref, _ = so.grid_classes(par["Mx"], par["My"], par["Mz"])
     assert np.array_equal(s.gt, gt_ref)
     for kind in (0, 1):
         groups, slots, loads, sources = s.plan_stats(kind)
         assert groups >= 1 and slots >= 1
-        # every source staged once per tile when the LDS holds the live set
         assert loads == sources
+    s.close()
+    # default (two workgroups per CU): re-staging stays bounded
+    monkeypatch.delenv("SMP_BLOCKS_PER_CU")
+    s = HipSolver(par, device=-1)
+    for kind in (0, 1):
+        groups, slots, loads, sources = s.plan_stats(kind)
+        assert sources <= loads <= 2 * sources
     from starmap_amd._lib import SmpError
     with pytest.raises(SmpError, match="without a device"):
         s.step(0.1, 1)
