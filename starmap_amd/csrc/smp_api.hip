@@ -216,6 +216,57 @@ struct smp_problem {
     }
 };
 
+namespace {
+
+// Build the schedule of one substep kind for a reserve of staged coefficient
+// fields.  Workgroups per CU (SMP_BLOCKS_PER_CU, default 2): fewer LDS slots
+// per workgroup trade source re-staging (mostly L2 / Infinity Cache hits) for
+// occupancy; when the slots are too few for the coupling the schedule falls
+// back to fewer workgroups.  Within the chosen budget the schedule shrinks to
+// the smallest slot count that stages every source once, when that fits.
+int make_plan(smp_problem *h, int kind, int ncf_reserve) {
+    std::string err = "no schedule";
+    const size_t slot_bytes = (size_t)h->ts.slot_elems * h->elem;
+    const size_t fixed = smp::substep_lds_overhead() + (size_t)ncf_reserve * 256 * h->elem;
+    const int max_load_elems = smp::MAXL * h->ts.slot_elems;
+    smp::Plan plan;
+    int blocks = h->blocks_per_cu;
+    int cap = 0;
+    for (; blocks >= 1; --blocks) {
+        const size_t per_block = 160 * 1024 / blocks;
+        if (per_block <= fixed + 2 * slot_bytes) continue;
+        cap = (int)((per_block - fixed) / slot_bytes);
+        err = smp::build_plan(h->geo, h->ts, kind, cap, smp::GMAX, max_load_elems, smp::MAXL, plan);
+        if (err.empty()) break;
+    }
+    if (!err.empty()) return fail(SMP_E_INVALID, err);
+    if ((int)plan.loads.size() == plan.n_sources) {
+        int lo = 2, hi = cap;
+        while (lo < hi) {
+            const int mid = (lo + hi) / 2;
+            smp::Plan trial;
+            std::string e2 = smp::build_plan(h->geo, h->ts, kind, mid, smp::GMAX, max_load_elems,
+                                             smp::MAXL, trial);
+            if (e2.empty() && (int)trial.loads.size() == trial.n_sources)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        err = smp::build_plan(h->geo, h->ts, kind, lo, smp::GMAX, max_load_elems, smp::MAXL, plan);
+        if (!err.empty()) return fail(SMP_E_INVALID, err);
+    }
+    h->plan[kind] = plan;
+    h->ncf_cap[kind] = ncf_reserve;
+    h->lds_bytes[kind] = fixed + (size_t)std::max(plan.n_slots, 1) * slot_bytes;
+    if (h->host_only) return SMP_OK;
+    // per-group record blobs, filled in whenever coefficients change
+    h->dfree(h->plan_dev[kind]);
+    h->plan_dev[kind] = nullptr;
+    return h->dalloc(&h->plan_dev[kind], std::max<size_t>(1, plan.groups.size()) * smp::BLOB_SZ);
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *smp_last_error(void) { return g_err.c_str(); }
@@ -289,72 +340,15 @@ int smp_create(const smp_desc *d, smp_problem **out) {
         h->ntile[e] = (h->E[e] + h->ts.tile[e] - 1) / h->ts.tile[e];
         h->n_tiles *= h->ntile[e];
     }
-    // schedules
-    // Workgroups per CU to aim for (SMP_BLOCKS_PER_CU overrides; default 2,
-    // the most the substep kernel's registers allow): fewer LDS slots per
-    // workgroup trade source re-staging (mostly L2 / Infinity Cache hits)
-    // for occupancy.  The schedule still shrinks to the smallest slot count
-    // that stages every source once when that fits.
+    // schedules (re-made at the first step once the staged coefficient
+    // fields are known)
     int want_blocks = 2;
     if (const char *env = std::getenv("SMP_BLOCKS_PER_CU")) want_blocks = std::atoi(env);
-    if (want_blocks < 0 || want_blocks > 8) return fail(SMP_E_INVALID, "SMP_BLOCKS_PER_CU must be 0..8");
+    if (want_blocks < 1 || want_blocks > 8) return fail(SMP_E_INVALID, "SMP_BLOCKS_PER_CU must be 1..8");
     h->blocks_per_cu = want_blocks;
     for (int kind = 0; kind < 2; ++kind) {
-        // coefficient fields staged per tile: decay rate and factor of each
-        // (grid type, order) among the targets, plus one source field
-        std::set<std::tuple<int, int, int, int>> keys;
-        const std::vector<int32_t> &tg = kind == 0 ? h->geo.odd : h->geo.even;
-        for (int j : tg)
-            keys.insert(std::make_tuple(h->geo.gt[3 * j], h->geo.gt[3 * j + 1], h->geo.gt[3 * j + 2],
-                                        j == 0 ? -1 : d->mom_order[j]));
-        h->ncf_cap[kind] = std::min<int>(smp::MAX_CFIELDS, 2 * (int)keys.size() + 1);
-        const size_t fixed = smp::substep_lds_overhead() +
-                             (size_t)h->ncf_cap[kind] * h->ts.threads * h->elem;
-        const size_t per_block = want_blocks > 0 ? 160 * 1024 / want_blocks : 160 * 1024;
-        if (per_block <= fixed + (size_t)h->ts.slot_elems * h->elem)
-            return fail(SMP_E_INVALID, "LDS too small for the requested workgroups per CU");
-        const int cap = (int)((per_block - fixed) / ((size_t)h->ts.slot_elems * h->elem));
-        err = smp::build_plan(h->geo, h->ts, kind, cap, smp::GMAX, smp::PF * h->ts.threads,
-                              smp::MAXL, h->plan[kind]);
-        if (!err.empty()) return fail(SMP_E_INVALID, err);
-        // the smallest slot count that still stages every source once per
-        // tile: less LDS per workgroup, more workgroups per CU
-        if ((int)h->plan[kind].loads.size() == h->plan[kind].n_sources) {
-            int lo = 1, hi = h->plan[kind].n_slots;
-            while (lo < hi) {
-                const int mid = (lo + hi) / 2;
-                smp::Plan trial;
-                std::string e2 = smp::build_plan(h->geo, h->ts, kind, mid, smp::GMAX,
-                                                 smp::PF * h->ts.threads, smp::MAXL, trial);
-                if (e2.empty() && (int)trial.loads.size() == trial.n_sources)
-                    hi = mid;
-                else
-                    lo = mid + 1;
-            }
-            err = smp::build_plan(h->geo, h->ts, kind, lo, smp::GMAX, smp::PF * h->ts.threads,
-                                  smp::MAXL, h->plan[kind]);
-            if (!err.empty()) return fail(SMP_E_INVALID, err);
-        }
-        h->lds_bytes[kind] = fixed + (size_t)std::max(h->plan[kind].n_slots, 1) *
-                                         h->ts.slot_elems * h->elem;
-        const smp::Plan &p = h->plan[kind];
-        if ((int)p.groups.size() > smp::MAX_GROUPS)
-            return fail(SMP_E_INVALID, "substep schedule has too many groups");
-        auto al = [](size_t v) { return (v + 255) / 256 * 256; };
-        size_t s0 = p.groups.size() * sizeof(smp::DevGroup);
-        size_t s1 = p.loads.size() * sizeof(smp::DevLoad);
-        size_t s2 = p.targets.size() * sizeof(smp::TgtRec);
-        h->plan_off[kind][0] = al(s0);
-        h->plan_off[kind][1] = al(s0) + al(s1);
-        h->plan_off[kind][2] = al(s0) + al(s1) + al(s2);
-        if (h->host_only) continue;
-        int rc = h->dalloc(&h->plan_dev[kind], h->plan_off[kind][2]);
+        int rc = make_plan(h.get(), kind, 0);
         if (rc) return rc;
-        std::vector<unsigned char> buf(h->plan_off[kind][2], 0);
-        std::memcpy(buf.data(), p.groups.data(), s0);
-        std::memcpy(buf.data() + h->plan_off[kind][0], p.loads.data(), s1);
-        std::memcpy(buf.data() + h->plan_off[kind][1], p.targets.data(), s2);
-        HIP_TRY(hipMemcpy(h->plan_dev[kind], buf.data(), buf.size(), hipMemcpyHostToDevice));
     }
     if (h->host_only) {
         *out = h.release();
@@ -587,6 +581,19 @@ int refresh_coefs(smp_problem *h) {
         }
     }
     for (int kind = 0; kind < 2; ++kind) {
+        // re-plan when the number of coefficient fields to stage per tile
+        // differs from the LDS the schedule reserved for them
+        std::set<const void *> distinct;
+        for (const smp::TgtRec &r : h->plan[kind].targets) {
+            const CF &c = cf[r.comp];
+            for (const Field *f : {c.sig, c.E, c.Q})
+                if (f && f->dev) distinct.insert(f->dev);
+        }
+        const int need = std::min<int>((int)distinct.size(), smp::MAX_CFIELDS);
+        if (need != h->ncf_cap[kind]) {
+            int rc = make_plan(h, kind, need);
+            if (rc) return rc;
+        }
         // coefficient fields of this kind's targets, staged per tile in LDS
         // up to the reserved count, read from global memory beyond it
         std::vector<const void *> table;
@@ -636,8 +643,19 @@ int refresh_coefs(smp_problem *h) {
         if (!table.empty())
             HIP_TRY(hipMemcpy(h->cf_dev[kind], table.data(), sizeof(void *) * table.size(),
                               hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy((char *)h->plan_dev[kind] + h->plan_off[kind][1], recs.data(),
-                          recs.size() * sizeof(smp::TgtRec), hipMemcpyHostToDevice));
+        const smp::Plan &p = h->plan[kind];
+        std::vector<unsigned char> blobs(std::max<size_t>(1, p.groups.size()) * smp::BLOB_SZ, 0);
+        for (size_t q = 0; q < p.groups.size(); ++q) {
+            const smp::DevGroup &gr = p.groups[q];
+            unsigned char *b = blobs.data() + q * smp::BLOB_SZ;
+            const int32_t hdr[4] = {gr.l1 - gr.l0, gr.t1 - gr.t0, 0, 0};
+            std::memcpy(b, hdr, sizeof(hdr));
+            std::memcpy(b + smp::BLOB_REC_OFF, recs.data() + gr.t0,
+                        sizeof(smp::TgtRec) * (gr.t1 - gr.t0));
+            std::memcpy(b + smp::BLOB_LD_OFF, p.loads.data() + gr.l0,
+                        sizeof(smp::DevLoad) * (gr.l1 - gr.l0));
+        }
+        HIP_TRY(hipMemcpy(h->plan_dev[kind], blobs.data(), blobs.size(), hipMemcpyHostToDevice));
     }
     h->coef_dirty = false;
     return SMP_OK;
@@ -664,12 +682,13 @@ smp::SubstepArgs<T> make_args(smp_problem *h, int kind, double dt, int napp, int
         a.lds[e] = h->ts.lds[e];
     }
     a.slot_elems = h->ts.slot_elems;
+    a.lds0_shift = 0;
+    while ((1 << a.lds0_shift) < h->ts.lds[0]) ++a.lds0_shift;
+    a.inv_lds1 = 1.0f / (float)h->ts.lds[1];
+    a.chunks = (h->ts.slot_elems * (h->elem / 4) + 63) / 64;
     a.n_tiles = h->n_tiles;
     const smp::Plan &p = h->plan[kind];
-    char *base = (char *)h->plan_dev[kind];
-    a.groups = (const smp::DevGroup *)base;
-    a.loads = (const smp::DevLoad *)(base + h->plan_off[kind][0]);
-    a.targets = (const smp::TgtRec *)(base + h->plan_off[kind][1]);
+    a.blobs = (const unsigned char *)h->plan_dev[kind];
     a.ngroups = (int)p.groups.size();
     a.cfields = (const void *const *)h->cf_dev[kind];
     a.ncf = h->ncf[kind];

@@ -13,6 +13,12 @@ constexpr int MAXL = 16;      // source loads per group
 constexpr int MAX_GROUPS = 256;
 constexpr int MAX_CFIELDS = 12; // coefficient fields staged per tile
 
+// Per-group record blob, packed by the host and copied into LDS by the
+// kernel: header {loads, targets}, GMAX target records, MAXL load records.
+constexpr int BLOB_SZ = 3072;
+constexpr int BLOB_REC_OFF = 16;
+constexpr int BLOB_LD_OFF = BLOB_REC_OFF + GMAX * (int)sizeof(TgtRec);
+
 template <typename T>
 struct SubstepArgs {
     T *U;
@@ -20,12 +26,13 @@ struct SubstepArgs {
     int32_t pitch0, ext1;     // storage index = i0 + pitch0*(i1 + ext1*i2)
     int32_t n[3], bc[3];      // per kernel axis
     int32_t tile[3], ntile[3], lds[3];
-    int32_t slot_elems;
+    int32_t slot_elems;       // lds[0]*lds[1]*lds[2]; lds[0] is a power of two
+    int32_t lds0_shift;       // log2(lds[0])
+    float inv_lds1;           // 1/lds[1]
+    int32_t chunks;           // DMA chunks (64 lanes x 4 bytes) per slot
     int32_t n_tiles;          // total tiles
     int32_t ngroups;
-    const DevGroup *groups;
-    const DevLoad *loads;
-    const TgtRec *targets;
+    const unsigned char *blobs;   // ngroups * BLOB_SZ bytes
     const void *const *cfields;   // coefficient fields staged per tile
     int32_t ncf;
     int32_t E[3];             // storage extents per kernel axis

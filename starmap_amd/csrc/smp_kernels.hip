@@ -40,201 +40,181 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 
 }  // namespace
 
-// LDS layout: group headers | 2 x (GMAX target records, MAXL load records) |
-// source slots
-constexpr int BLOB_REC = GMAX * (int)sizeof(TgtRec);
-constexpr int BLOB = BLOB_REC + MAXL * (int)sizeof(DevLoad);
+static_assert(BLOB_LD_OFF + MAXL * (int)sizeof(DevLoad) <= BLOB_SZ, "blob layout");
+static_assert(sizeof(DevLoad) == 32 && sizeof(TgtRec) % 16 == 0, "record layout");
 
-size_t substep_lds_overhead() { return MAX_GROUPS * sizeof(DevGroup) + 2 * BLOB; }
+// LDS: a ring of 3 group blobs | staged coefficient fields | source slots
+size_t substep_lds_overhead() { return 3 * (size_t)BLOB_SZ; }
 
-// One workgroup = one spatial tile, one thread = one cell.  While the targets
-// of group g are updated from LDS and registers, the source tiles and target
-// values of group g+1 and the records of group g+2 are in flight; the update
-// itself waits on no global load, so the in-order load counter never makes
-// it wait for the prefetch.  The coefficient fields the targets use (decay
-// rates, decay factors, sources) are staged per tile once.
+typedef __attribute__((address_space(3))) void *lds_ptr_t;
+
+// One workgroup = one spatial tile of up to 256 cells, one thread per cell.
+// Source tiles go from HBM straight into LDS slots (global_load_lds, no
+// registers), the group records the same way.  While the targets of group g
+// are updated, the source tiles and target values of group g+1 and the
+// records of group g+2 are in flight; one barrier per group.  The schedule
+// never lets group g+1 copy into a slot that group g reads.
 template <typename T, int NT>
 __global__ __launch_bounds__(256) void substep_kernel(const SubstepArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    DevGroup *grp = reinterpret_cast<DevGroup *>(smem_raw);
-    unsigned char *blob = smem_raw + MAX_GROUPS * sizeof(DevGroup);
-    T *cfa = reinterpret_cast<T *>(blob + 2 * BLOB);
-    const int nthr = blockDim.x;
-    T *lds = cfa + a.ncf * nthr;
-    constexpr int RECV = sizeof(TgtRec) / 16;
-    constexpr int LDV = sizeof(DevLoad) / 16;   // DevLoad is 32 bytes
-    static_assert(sizeof(DevLoad) == 32, "DevLoad layout");
-    static_assert(GMAX * RECV + MAXL * LDV <= 256, "one blob chunk per thread");
+    unsigned char *ring = smem_raw;
+    T *cfa = reinterpret_cast<T *>(smem_raw + 3 * BLOB_SZ);
+    T *lds = cfa + a.ncf * 256;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= a.n_tiles) return;
     const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
     int b = tile;
     const int t0 = b % a.ntile[0];
     b /= a.ntile[0];
     const int t1 = b % a.ntile[1];
     const int t2 = b / a.ntile[1];
     const int o0 = t0 * a.tile[0], o1 = t1 * a.tile[1], o2 = t2 * a.tile[2];
+    const int ncell = a.tile[0] * a.tile[1] * a.tile[2];
+    const bool has_cell = tid < ncell;
     const int c0 = tid % a.tile[0];
     const int c1 = (tid / a.tile[0]) % a.tile[1];
     const int c2 = tid / (a.tile[0] * a.tile[1]);
     const int g0 = o0 + c0, g1 = o1 + c1, g2 = o2 + c2;
-    const bool in_store = g0 < a.E[0] && g1 < a.E[1] && g2 < a.E[2];
+    const bool in_store = has_cell && g0 < a.E[0] && g1 < a.E[1] && g2 < a.E[2];
     const int lbase = c0 + a.lds[0] * (c1 + a.lds[1] * c2);
     const int64_t gidx = g0 + (int64_t)a.pitch0 * (g1 + (int64_t)a.ext1 * g2);
     const int S = a.slot_elems;
-    const int L0 = a.lds[0], L1 = a.lds[1];
+    const int L1 = a.lds[1];
     const T half_dt = (T)a.half_dt;
     const int ng = a.ngroups;
+    // elements per DMA chunk and dwords per element
+    constexpr int DPE = sizeof(T) / 4;
+    constexpr int EPC = 64 / DPE;
 
-    for (int i = tid; i < ng; i += nthr) grp[i] = a.groups[i];
     for (int c = 0; c < a.ncf; ++c)
-        cfa[c * nthr + tid] = in_store ? reinterpret_cast<const T *>(a.cfields[c])[gidx] : (T)0;
+        cfa[c * 256 + tid] = in_store ? reinterpret_cast<const T *>(a.cfields[c])[gidx] : (T)0;
 
-    // the staged elements of this thread: element e = tid + k*nthr of a
-    // group's load list is slot-local element r of load li; its tile-local
-    // coordinates are fixed for the whole kernel
-    int epack[PF];
-#pragma unroll
-    for (int k = 0; k < PF; ++k) {
-        const int e = tid + k * nthr;
-        const int li = e / S;
-        const int r = e - li * S;
-        const int l0 = r % L0;
-        const int rr = r / L0;
-        // l0 < 1024, l1 < 128, l2 < 128, li < 128 (checked by the host)
-        epack[k] = l0 | ((rr % L1) << 10) | ((rr / L1) << 17) | (li << 24);
-    }
-
-    T pf[PF];
-    int pdst[PF];
-    int4 bpf = make_int4(0, 0, 0, 0);
-    T u_cur[GMAX], u_nxt[GMAX];
-
-    auto recs = [&](int buf) { return reinterpret_cast<TgtRec *>(blob + buf * BLOB); };
-    auto ldr = [&](int buf) { return reinterpret_cast<DevLoad *>(blob + buf * BLOB + BLOB_REC); };
-
-    // global -> registers: the source tiles of a group (load records in LDS)
-    auto fetch_src = [&](const DevGroup &gr, int buf) {
-        const int nl = gr.l1 - gr.l0;
-        const DevLoad *L = ldr(buf);
-#pragma unroll
-        for (int k = 0; k < PF; ++k) {
-            const int li = epack[k] >> 24;
-            pdst[k] = -1;
-            if (li < nl) {
+    auto hdr_nl = [&](int buf) { return reinterpret_cast<const int *>(ring + buf * BLOB_SZ)[0]; };
+    auto hdr_nt = [&](int buf) { return reinterpret_cast<const int *>(ring + buf * BLOB_SZ)[1]; };
+    auto recs = [&](int buf) {
+        return reinterpret_cast<const TgtRec *>(ring + buf * BLOB_SZ + BLOB_REC_OFF);
+    };
+    auto ldrec = [&](int buf) {
+        return reinterpret_cast<const DevLoad *>(ring + buf * BLOB_SZ + BLOB_LD_OFF);
+    };
+    // HBM -> LDS: the blob of group gq (three 1 KB chunks, 16 bytes a lane)
+    auto blob_dma = [&](int gq, int buf) {
+        if (wave < 3) {
+            const unsigned char *src = a.blobs + (size_t)gq * BLOB_SZ + wave * 1024 + lane * 16;
+            __builtin_amdgcn_global_load_lds(src, (lds_ptr_t)(ring + buf * BLOB_SZ + wave * 1024),
+                                             16, 0, 0);
+        }
+    };
+    // HBM -> LDS: the source tiles of the group in blob buf, 4 bytes a lane;
+    // chunk c of the group is slot-local dword range [64*cc, 64*cc+64) of
+    // load li, copied by wave c % 4
+    auto src_dma = [&](int buf) {
+        const int nl = hdr_nl(buf);
+        const DevLoad *L = ldrec(buf);
+        const int cps = a.chunks;
+        const int total = nl * cps;
+        for (int c = wave; c < total; c += 4) {
+            const int li = c / cps;
+            const int cc = c - li * cps;
+            const int e = cc * EPC + lane / DPE;
+            if (e < S) {
                 const DevLoad &ld = L[li];
-                const int l0 = epack[k] & 1023, l1 = (epack[k] >> 10) & 127,
-                          l2 = (epack[k] >> 17) & 127;
+                const int l0 = e & (a.lds[0] - 1);
+                const int row = e >> a.lds0_shift;
+                const int l2 = (int)(((float)row + 0.5f) * a.inv_lds1);
+                const int l1 = row - l2 * L1;
                 const int x0 = wrap_or_clamp(o0 + l0 + ld.org[0], a.n[0], a.bc[0], ld.ext[0]);
                 const int x1 = wrap_or_clamp(o1 + l1 + ld.org[1], a.n[1], a.bc[1], ld.ext[1]);
                 const int x2 = wrap_or_clamp(o2 + l2 + ld.org[2], a.n[2], a.bc[2], ld.ext[2]);
-                pf[k] = a.U[(int64_t)ld.comp * a.slab + x0 +
-                            (int64_t)a.pitch0 * (x1 + (int64_t)a.ext1 * x2)];
-                pdst[k] = ld.slot * S + l0 + L0 * (l1 + L1 * l2);
+                const T *src = a.U + (int64_t)ld.comp * a.slab + x0 +
+                               (int64_t)a.pitch0 * (x1 + (int64_t)a.ext1 * x2);
+                const int slot = __builtin_amdgcn_readfirstlane(ld.slot);
+                unsigned char *dst = reinterpret_cast<unsigned char *>(lds + slot * S) + cc * 256;
+                __builtin_amdgcn_global_load_lds(
+                    reinterpret_cast<const unsigned *>(src) + (DPE == 2 ? (lane & 1) : 0),
+                    (lds_ptr_t)dst, 4, 0, 0);
             }
         }
     };
-    auto commit_src = [&]() {
-#pragma unroll
-        for (int k = 0; k < PF; ++k)
-            if (pdst[k] >= 0) lds[pdst[k]] = pf[k];
-    };
-    // global -> registers -> LDS: target and load records of a group
-    auto fetch_blob = [&](const DevGroup &gr) {
-        const int nr = (gr.t1 - gr.t0) * RECV;
-        const int nl = (gr.l1 - gr.l0) * LDV;
-        if (tid < nr)
-            bpf = reinterpret_cast<const int4 *>(a.targets + gr.t0)[tid];
-        else if (tid >= GMAX * RECV && tid < GMAX * RECV + nl)
-            bpf = reinterpret_cast<const int4 *>(a.loads + gr.l0)[tid - GMAX * RECV];
-    };
-    auto commit_blob = [&](const DevGroup &gr, int buf) {
-        const int nr = (gr.t1 - gr.t0) * RECV;
-        const int nl = (gr.l1 - gr.l0) * LDV;
-        if (tid < nr || (tid >= GMAX * RECV && tid < GMAX * RECV + nl))
-            reinterpret_cast<int4 *>(blob + buf * BLOB)[tid] = bpf;
-    };
-    // global -> registers: the target values of a group
-    auto load_targets = [&](const DevGroup &gr, int buf, T *u) {
+    // HBM -> registers: the target values of the group in blob buf
+    auto load_targets = [&](int buf, T *u) {
         const TgtRec *r = recs(buf);
-        const int nt = gr.t1 - gr.t0;
+        const int nt = __builtin_amdgcn_readfirstlane(hdr_nt(buf));
 #pragma unroll
         for (int i = 0; i < GMAX; ++i) {
             if (i < nt) {
-                const TgtRec &ri = r[i];
-                if (g0 < ri.ext[0] && g1 < ri.ext[1] && g2 < ri.ext[2])
-                    u[i] = a.U[(int64_t)ri.comp * a.slab + gidx];
+                const int4 h0 = reinterpret_cast<const int4 *>(r + i)[0];   // comp ..
+                const int4 h1 = reinterpret_cast<const int4 *>(r + i)[1];   // Q_off, ext[3]
+                if (has_cell && g0 < h1.y && g1 < h1.z && g2 < h1.w)
+                    u[i] = a.U[(int64_t)h0.x * a.slab + gidx];
             }
         }
     };
     auto coef = [&](int off, const void *p, double s) -> T {
         if (off >= 0) return cfa[off + tid];
         if (off == -1) return (T)s;
-        return reinterpret_cast<const T *>(p)[gidx];
+        return in_store ? reinterpret_cast<const T *>(p)[gidx] : (T)0;
     };
 
+    T u_cur[GMAX], u_nxt[GMAX];
+    if (ng > 0) blob_dma(0, 0);
+    if (ng > 1) blob_dma(1, 1);
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (ng > 0) {
-        fetch_blob(grp[0]);
-        commit_blob(grp[0], 0);
-        if (ng > 1) {
-            fetch_blob(grp[1]);
-            commit_blob(grp[1], 1);
-        }
+        src_dma(0);
+        load_targets(0, u_cur);
     }
-    __syncthreads();
-    if (ng > 0) {
-        fetch_src(grp[0], 0);
-        commit_src();
-        load_targets(grp[0], 0, u_cur);
-    }
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
 
     for (int gq = 0; gq < ng; ++gq) {
-        const DevGroup gr = grp[gq];
-        const int buf = gq & 1;
+        const int buf = gq % 3;
         if (gq + 1 < ng) {
-            load_targets(grp[gq + 1], buf ^ 1, u_nxt);
-            fetch_src(grp[gq + 1], buf ^ 1);
+            src_dma((gq + 1) % 3);
+            load_targets((gq + 1) % 3, u_nxt);
         }
-        if (gq + 2 < ng) fetch_blob(grp[gq + 2]);
+        if (gq + 2 < ng) blob_dma(gq + 2, (gq + 2) % 3);
         const TgtRec *rb = recs(buf);
-        const int nt = gr.t1 - gr.t0;
+        const int nt = __builtin_amdgcn_readfirstlane(hdr_nt(buf));
 #pragma unroll
         for (int i = 0; i < GMAX; ++i) {
             if (i < nt) {
-                const TgtRec &r = rb[i];
-                if (g0 < r.ext[0] && g1 < r.ext[1] && g2 < r.ext[2]) {
-                    // W = -sumcell(...): the coefficients carry -M(j,k)/h
-                    T w = 0;
-                    if (r.nterm > 0) {
+                // one batch of wide LDS reads for the whole record
+                const TgtRec r = rb[i];
+                const bool act = has_cell && g0 < r.ext[0] && g1 < r.ext[1] && g2 < r.ext[2];
+                // W = -sumcell(...): the coefficients carry -M(j,k)/h;
+                // padding terms add coef 0 times a zero difference
+                T w = 0;
+                if (__builtin_amdgcn_readfirstlane(r.nterm) > 0) {
 #pragma unroll
-                        for (int q = 0; q < NT; ++q) {
-                            const Term tm = r.t[q];
-                            const int o = tm.off + lbase;
-                            w += (T)tm.coef * (lds[o + tm.stride] - lds[o]);
-                        }
+                    for (int q = 0; q < NT; ++q) {
+                        const int o = r.t[q].off + lbase;
+                        w += (T)r.t[q].coef * (lds[o + r.t[q].stride] - lds[o]);
                     }
-                    const T sig = coef(r.sig_off, r.sig_p, r.sig_s);
-                    const T E = coef(r.E_off, r.E_p, r.E_s);
-                    const T wq = r.Q_off == -1 ? w : w + coef(r.Q_off, r.Q_p, 0.0);
-                    T u = u_cur[i];
-                    for (int app = 0; app < a.napp; ++app) {
-                        // U + dt/2*(W+Q - s.*U).*E   (starmap_solver.m:352-354)
-                        u = u + (half_dt * (wq - sig * u)) * E;
-                        if (r.filt) {
-                            const int s = a.fslot[app];
-                            u = u * (r.F[s] ? reinterpret_cast<const T *>(r.F[s])[gidx]
-                                            : (T)r.F_s[s]);
-                        }
-                    }
-                    a.U[(int64_t)r.comp * a.slab + gidx] = u;
                 }
+                const T sig = coef(r.sig_off, r.sig_p, r.sig_s);
+                const T E = coef(r.E_off, r.E_p, r.E_s);
+                const T wq = r.Q_off == -1 ? w : w + coef(r.Q_off, r.Q_p, 0.0);
+                T u = u_cur[i];
+                for (int app = 0; app < a.napp; ++app) {
+                    // U + dt/2*(W+Q - s.*U).*E   (starmap_solver.m:352-354)
+                    u = u + (half_dt * (wq - sig * u)) * E;
+                    if (r.filt) {
+                        const bool s1 = a.fslot[app] != 0;
+                        const void *fp = s1 ? r.F[1] : r.F[0];
+                        const double fs = s1 ? r.F_s[1] : r.F_s[0];
+                        u = u * (fp && act ? reinterpret_cast<const T *>(fp)[gidx] : (T)fs);
+                    }
+                }
+                if (act) a.U[(int64_t)r.comp * a.slab + gidx] = u;
             }
         }
-        __syncthreads();
-        if (gq + 1 < ng) commit_src();
-        if (gq + 2 < ng) commit_blob(grp[gq + 2], buf);
+        // the copies of group g+1 (and its target values) must have landed
+        // and every wave must be done with group g before anyone goes on
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < GMAX; ++i) u_cur[i] = u_nxt[i];
@@ -260,7 +240,7 @@ static hipError_t launch_nt(const SubstepArgs<T> &a, int threads, size_t lds_byt
 template <typename T>
 hipError_t launch_substep(const SubstepArgs<T> &a, int nt, int threads, size_t lds_bytes,
                           hipStream_t s) {
-    if (a.ngroups > MAX_GROUPS || a.ncf > MAX_CFIELDS) return hipErrorInvalidValue;
+    if (a.ncf > MAX_CFIELDS || threads != 256) return hipErrorInvalidValue;
     if (nt <= 4) return launch_nt<T, 4>(a, threads, lds_bytes, s);
     if (nt <= 8) return launch_nt<T, 8>(a, threads, lds_bytes, s);
     return launch_nt<T, MAXT>(a, threads, lds_bytes, s);

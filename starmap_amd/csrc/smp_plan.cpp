@@ -89,7 +89,7 @@ TileShape make_tile(const Geometry &g, int tx, int ty, int tz) {
     TileShape ts;
     int t[3] = {tx, ty, tz};
     for (int e = 0; e < 3; ++e) ts.tile[e] = e < g.ndim ? t[e] : 1;
-    ts.threads = ts.tile[0] * ts.tile[1] * ts.tile[2];
+    ts.threads = 256;
     for (int e = 0; e < 3; ++e) ts.lds[e] = e < g.ndim ? ts.tile[e] + 1 : 1;
     ts.slot_elems = ts.lds[0] * ts.lds[1] * ts.lds[2];
     return ts;
@@ -97,9 +97,11 @@ TileShape make_tile(const Geometry &g, int tx, int ty, int tz) {
 
 TileShape choose_tile(const Geometry &g, int elem_bytes) {
     (void)elem_bytes;
-    if (g.ndim == 1) return make_tile(g, 256, 1, 1);
-    if (g.ndim == 2) return make_tile(g, 64, 4, 1);
-    return make_tile(g, 32, 4, 2);
+    // the first axis spans a power of two with its halo (LDS rows of 256, 64
+    // or 32 elements); 256 threads, one per cell, spare threads only copy
+    if (g.ndim == 1) return make_tile(g, 255, 1, 1);
+    if (g.ndim == 2) return make_tile(g, 63, 4, 1);
+    return make_tile(g, 31, 4, 2);
 }
 
 std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slot_capacity,
@@ -131,7 +133,10 @@ std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slo
     }
     out.n_sources = (int)all_src.size();
 
-    // groups of consecutive targets whose joint sources fit the capacity
+    // groups of consecutive targets; a group's tiles are copied in while the
+    // previous group is still read, so two consecutive groups must fit
+    // together: each group gets half of the slots
+    const int group_cap = slot_capacity / 2;
     std::vector<std::pair<int, int>> groups;
     std::vector<std::vector<int>> gsrc;
     for (size_t t = 0; t < tg.size();) {
@@ -140,10 +145,11 @@ std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slo
         while (t1 < tg.size() && (int)(t1 - t) < group_size) {
             std::set<int> s2 = s;
             s2.insert(src[t1].begin(), src[t1].end());
-            if ((int)s2.size() > slot_capacity) break;
+            if ((int)s2.size() > group_cap) break;
             s = s2;
             ++t1;
         }
+        if (t1 == t) return "a target couples to more components than half the LDS slots";
         groups.push_back({(int)t, (int)t1});
         gsrc.push_back(std::vector<int>(s.begin(), s.end()));
         t = t1;
@@ -172,6 +178,10 @@ std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slo
                 for (int c = 0; c < slot_capacity; ++c) {
                     int o = owner[c];
                     if (std::binary_search(gsrc[q].begin(), gsrc[q].end(), o)) continue;
+                    // the previous group still reads its sources while this
+                    // group's tiles are copied in
+                    if (q > 0 && std::binary_search(gsrc[q - 1].begin(), gsrc[q - 1].end(), o))
+                        continue;
                     int nu = next_use(o, q + 1);
                     if (nu > far) { far = nu; best = c; }
                 }

@@ -87,9 +87,10 @@ struct TileShape {
     int lds[3] = {1, 1, 1};      // tile + 1 halo per axis
     int slot_elems = 1;
     int threads = 256;
+    int cells() const { return tile[0] * tile[1] * tile[2]; }
 };
 
-// tile of tx*ty*tz cells (256 threads, one cell each)
+// tile of tx*ty*tz <= 256 cells (256 threads, one cell each)
 TileShape make_tile(const Geometry &g, int tx, int ty, int tz);
 TileShape choose_tile(const Geometry &g, int elem_bytes);
 
