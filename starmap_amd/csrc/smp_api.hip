@@ -84,6 +84,7 @@ struct smp_problem {
     size_t lds_bytes[2] = {0, 0};
     int ncf_cap[2] = {0, 0};       // coefficient fields staged per tile (LDS reserved)
     int ncf[2] = {0, 0};           // coefficient fields currently staged
+    bool gen[2] = {false, false};  // some coefficient is read from global memory
     void *cf_dev[2] = {nullptr, nullptr};
     int blocks_per_cu = 0;
     bool host_only = false;
@@ -219,7 +220,7 @@ struct smp_problem {
 namespace {
 
 // Build the schedule of one substep kind for a reserve of staged coefficient
-// fields.  Workgroups per CU (SMP_BLOCKS_PER_CU, default 2): fewer LDS slots
+// fields.  Workgroups per CU (SMP_BLOCKS_PER_CU, default 1): fewer LDS slots
 // per workgroup trade source re-staging (mostly L2 / Infinity Cache hits) for
 // occupancy; when the slots are too few for the coupling the schedule falls
 // back to fewer workgroups.  Within the chosen budget the schedule shrinks to
@@ -342,7 +343,7 @@ int smp_create(const smp_desc *d, smp_problem **out) {
     }
     // schedules (re-made at the first step once the staged coefficient
     // fields are known)
-    int want_blocks = 2;
+    int want_blocks = 1;
     if (const char *env = std::getenv("SMP_BLOCKS_PER_CU")) want_blocks = std::atoi(env);
     if (want_blocks < 1 || want_blocks > 8) return fail(SMP_E_INVALID, "SMP_BLOCKS_PER_CU must be 1..8");
     h->blocks_per_cu = want_blocks;
@@ -636,6 +637,9 @@ int refresh_coefs(smp_problem *h) {
             r.F_s[1] = c.F_s[1];
         }
         h->ncf[kind] = (int)table.size();
+        h->gen[kind] = false;
+        for (const smp::TgtRec &r : recs)
+            if (r.sig_off == -2 || r.E_off == -2 || r.Q_off == -2 || r.F[0] || r.F[1]) h->gen[kind] = true;
         if (!h->cf_dev[kind]) {
             int rc = h->dalloc(&h->cf_dev[kind], sizeof(void *) * smp::MAX_CFIELDS);
             if (rc) return rc;
@@ -756,7 +760,7 @@ int run_steps(smp_problem *h, double dt, int64_t n_steps) {
         }
         const int kind = p == &even ? 1 : 0;
         if (li < timed) HIP_TRY(hipEventRecord(h->events[2 + 2 * li], h->stream));
-        HIP_TRY(smp::launch_substep<T>(*p, h->plan[kind].max_terms, h->ts.threads,
+        HIP_TRY(smp::launch_substep<T>(*p, h->plan[kind].max_terms, h->gen[kind], smp::KTHREADS,
                                        h->lds_bytes[kind], h->stream));
         if (li < timed) HIP_TRY(hipEventRecord(h->events[3 + 2 * li], h->stream));
         ++li;

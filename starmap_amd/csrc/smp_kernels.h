@@ -9,7 +9,7 @@ namespace smp {
 
 constexpr int GMAX = 8;       // targets per group
 constexpr int PF = 16;        // staged source elements per thread per group
-constexpr int MAXL = 16;      // source loads per group
+constexpr int MAXL = 12;      // source loads per group
 constexpr int MAX_GROUPS = 256;
 constexpr int MAX_CFIELDS = 12; // coefficient fields staged per tile
 
@@ -18,6 +18,8 @@ constexpr int MAX_CFIELDS = 12; // coefficient fields staged per tile
 constexpr int BLOB_SZ = 3072;
 constexpr int BLOB_REC_OFF = 16;
 constexpr int BLOB_LD_OFF = BLOB_REC_OFF + GMAX * (int)sizeof(TgtRec);
+constexpr int CPS_MAX = 16;   // DMA chunks per LDS slot
+constexpr int KTHREADS = 384; // 4 cell waves + 2 copy waves
 
 template <typename T>
 struct SubstepArgs {
@@ -45,10 +47,12 @@ struct SubstepArgs {
 // coefficient fields (ncf * threads elements)
 size_t substep_lds_overhead();
 
-// nt: terms per target the kernel unrolls (4, 8 or 12)
+// nt: terms per target the kernel unrolls (8 or 12); gen: some coefficient
+// is a field read from global memory (more fields than the LDS stages, or a
+// spatially varying filter)
 template <typename T>
-hipError_t launch_substep(const SubstepArgs<T> &a, int nt, int threads, size_t lds_bytes,
-                          hipStream_t s);
+hipError_t launch_substep(const SubstepArgs<T> &a, int nt, bool gen, int threads,
+                          size_t lds_bytes, hipStream_t s);
 
 template <typename T>
 hipError_t launch_decay(const T *sig, T *E, int64_t count, double dt, hipStream_t s);

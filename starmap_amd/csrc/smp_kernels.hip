@@ -48,14 +48,17 @@ size_t substep_lds_overhead() { return 3 * (size_t)BLOB_SZ; }
 
 typedef __attribute__((address_space(3))) void *lds_ptr_t;
 
-// One workgroup = one spatial tile of up to 256 cells, one thread per cell.
-// Source tiles go from HBM straight into LDS slots (global_load_lds, no
-// registers), the group records the same way.  While the targets of group g
-// are updated, the source tiles and target values of group g+1 and the
-// records of group g+2 are in flight; one barrier per group.  The schedule
-// never lets group g+1 copy into a slot that group g reads.
-template <typename T, int NT>
-__global__ __launch_bounds__(256) void substep_kernel(const SubstepArgs<T> a) {
+// One workgroup = one spatial tile of up to 256 cells.  Waves 0-3 own one
+// cell per thread and update the targets; waves 4-5 copy: source tiles and the
+// group records go from HBM straight into LDS (global_load_lds, no
+// registers).  While the cell waves update the targets of group g from LDS,
+// the copy wave brings in the source tiles of group g+1 and the records of
+// group g+2, and the cell waves load the target values of group g+1; one
+// barrier per group.  Only the copy waves have copies in flight, so the cell
+// waves never wait on them before reading LDS.  The schedule never lets group
+// g+1 copy into a slot that group g reads.
+template <typename T, int NT, int NAPP, bool GEN>
+__global__ __launch_bounds__(KTHREADS) void substep_kernel(const SubstepArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     unsigned char *ring = smem_raw;
     T *cfa = reinterpret_cast<T *>(smem_raw + 3 * BLOB_SZ);
@@ -64,7 +67,7 @@ __global__ __launch_bounds__(256) void substep_kernel(const SubstepArgs<T> a) {
     if (tile >= a.n_tiles) return;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wave = tid >> 6;
+    const bool copier = tid >= 256;
     int b = tile;
     const int t0 = b % a.ntile[0];
     b /= a.ntile[0];
@@ -81,15 +84,10 @@ __global__ __launch_bounds__(256) void substep_kernel(const SubstepArgs<T> a) {
     const int lbase = c0 + a.lds[0] * (c1 + a.lds[1] * c2);
     const int64_t gidx = g0 + (int64_t)a.pitch0 * (g1 + (int64_t)a.ext1 * g2);
     const int S = a.slot_elems;
-    const int L1 = a.lds[1];
     const T half_dt = (T)a.half_dt;
     const int ng = a.ngroups;
-    // elements per DMA chunk and dwords per element
-    constexpr int DPE = sizeof(T) / 4;
-    constexpr int EPC = 64 / DPE;
-
-    for (int c = 0; c < a.ncf; ++c)
-        cfa[c * 256 + tid] = in_store ? reinterpret_cast<const T *>(a.cfields[c])[gidx] : (T)0;
+    constexpr int DPE = sizeof(T) / 4;      // dwords per element
+    constexpr int EPC = 64 / DPE;           // elements per 64-lane DMA chunk
 
     auto hdr_nl = [&](int buf) { return reinterpret_cast<const int *>(ring + buf * BLOB_SZ)[0]; };
     auto hdr_nt = [&](int buf) { return reinterpret_cast<const int *>(ring + buf * BLOB_SZ)[1]; };
@@ -99,151 +97,220 @@ __global__ __launch_bounds__(256) void substep_kernel(const SubstepArgs<T> a) {
     auto ldrec = [&](int buf) {
         return reinterpret_cast<const DevLoad *>(ring + buf * BLOB_SZ + BLOB_LD_OFF);
     };
-    // HBM -> LDS: the blob of group gq (three 1 KB chunks, 16 bytes a lane)
-    auto blob_dma = [&](int gq, int buf) {
-        if (wave < 3) {
-            const unsigned char *src = a.blobs + (size_t)gq * BLOB_SZ + wave * 1024 + lane * 16;
-            __builtin_amdgcn_global_load_lds(src, (lds_ptr_t)(ring + buf * BLOB_SZ + wave * 1024),
-                                             16, 0, 0);
-        }
-    };
-    // HBM -> LDS: the source tiles of the group in blob buf, 4 bytes a lane;
-    // chunk c of the group is slot-local dword range [64*cc, 64*cc+64) of
-    // load li, copied by wave c % 4
-    auto src_dma = [&](int buf) {
-        const int nl = hdr_nl(buf);
-        const DevLoad *L = ldrec(buf);
-        const int cps = a.chunks;
-        const int total = nl * cps;
-        for (int c = wave; c < total; c += 4) {
-            const int li = c / cps;
-            const int cc = c - li * cps;
+
+    // ---- copy wave -------------------------------------------------------
+    // tile-local coordinates and, for tiles away from the domain boundary,
+    // the slab offset of the element each lane copies in chunk cc
+    bool interior = true;
+    for (int e = 0; e < 3; ++e) {
+        const int o = e == 0 ? o0 : e == 1 ? o1 : o2;
+        if (a.lds[e] > 1 && (o < 1 || o + a.lds[e] > a.n[e])) interior = false;
+    }
+    const int64_t tile_base = o0 + (int64_t)a.pitch0 * (o1 + (int64_t)a.ext1 * o2);
+    int pl[CPS_MAX];   // packed l0 | l1 << 10 | l2 << 17 of the lane's element, -1 past the slot
+    if (copier) {
+#pragma unroll
+        for (int cc = 0; cc < CPS_MAX; ++cc) {
             const int e = cc * EPC + lane / DPE;
-            if (e < S) {
-                const DevLoad &ld = L[li];
+            pl[cc] = -1;
+            if (cc < a.chunks && e < S) {
                 const int l0 = e & (a.lds[0] - 1);
                 const int row = e >> a.lds0_shift;
                 const int l2 = (int)(((float)row + 0.5f) * a.inv_lds1);
-                const int l1 = row - l2 * L1;
-                const int x0 = wrap_or_clamp(o0 + l0 + ld.org[0], a.n[0], a.bc[0], ld.ext[0]);
-                const int x1 = wrap_or_clamp(o1 + l1 + ld.org[1], a.n[1], a.bc[1], ld.ext[1]);
-                const int x2 = wrap_or_clamp(o2 + l2 + ld.org[2], a.n[2], a.bc[2], ld.ext[2]);
-                const T *src = a.U + (int64_t)ld.comp * a.slab + x0 +
-                               (int64_t)a.pitch0 * (x1 + (int64_t)a.ext1 * x2);
-                const int slot = __builtin_amdgcn_readfirstlane(ld.slot);
-                unsigned char *dst = reinterpret_cast<unsigned char *>(lds + slot * S) + cc * 256;
-                __builtin_amdgcn_global_load_lds(
-                    reinterpret_cast<const unsigned *>(src) + (DPE == 2 ? (lane & 1) : 0),
-                    (lds_ptr_t)dst, 4, 0, 0);
+                const int l1 = row - l2 * a.lds[1];
+                pl[cc] = l0 | (l1 << 10) | (l2 << 17);
+            }
+        }
+    }
+    auto blob_dma = [&](int gq, int buf) {
+        // three 1 KB chunks, 16 bytes a lane, by the first copy wave
+        if (tid >= 320) return;
+#pragma unroll
+        for (int k = 0; k < BLOB_SZ / 1024; ++k) {
+            const unsigned char *src = a.blobs + (size_t)gq * BLOB_SZ + k * 1024 + lane * 16;
+            __builtin_amdgcn_global_load_lds(src, (lds_ptr_t)(ring + buf * BLOB_SZ + k * 1024), 16, 0,
+                                             0);
+        }
+    };
+    auto src_dma = [&](int buf) {
+        // one load record per lane, read before the first copy is issued (an
+        // LDS read behind an outstanding copy would have to wait for it):
+        // comp | slot << 20 | (origin-shift bits) << 28
+        const int nl = __builtin_amdgcn_readfirstlane(hdr_nl(buf));
+        const int cw = (tid >> 6) - 4;   // copy wave 0 or 1: loads of its parity
+        int myrec = 0;
+        if (lane < nl) {
+            const DevLoad &d = ldrec(buf)[lane];
+            myrec = d.comp | (d.slot << 20) |
+                    (((d.org[0] & 1) | ((d.org[1] & 1) << 1) | ((d.org[2] & 1) << 2)) << 28);
+        }
+        for (int li = cw; li < nl; li += 2) {
+            const int rec = __builtin_amdgcn_readlane(myrec, li);
+            const int comp = rec & 0xFFFFF;
+            const int slot = (rec >> 20) & 0xFF;
+            const int m = (rec >> 28) & 7;   // bit e: -1 origin shift (cell-centred source)
+            const int s0 = -(m & 1), s1 = -((m >> 1) & 1), s2 = -((m >> 2) & 1);
+            const T *cb = a.U + (int64_t)comp * a.slab;
+            unsigned char *dst = reinterpret_cast<unsigned char *>(lds + slot * S);
+#pragma unroll
+            for (int cc = 0; cc < CPS_MAX; ++cc) {
+                if (cc < a.chunks) {
+                    const int l0 = pl[cc] & 1023, l1 = (pl[cc] >> 10) & 127, l2 = (pl[cc] >> 17) & 127;
+                    int64_t off;
+                    if (interior) {
+                        off = tile_base + (l0 + s0) +
+                              (int64_t)a.pitch0 * ((l1 + s1) + (int64_t)a.ext1 * (l2 + s2));
+                    } else {
+                        const int x0 = wrap_or_clamp(o0 + l0 + s0, a.n[0], a.bc[0],
+                                                     s0 ? a.n[0] : a.n[0] + a.bc[0]);
+                        const int x1 = wrap_or_clamp(o1 + l1 + s1, a.n[1], a.bc[1],
+                                                     s1 ? a.n[1] : a.n[1] + a.bc[1]);
+                        const int x2 = wrap_or_clamp(o2 + l2 + s2, a.n[2], a.bc[2],
+                                                     s2 ? a.n[2] : a.n[2] + a.bc[2]);
+                        off = x0 + (int64_t)a.pitch0 * (x1 + (int64_t)a.ext1 * x2);
+                    }
+                    const unsigned *src = reinterpret_cast<const unsigned *>(cb + off) +
+                                          (DPE == 2 ? (lane & 1) : 0);
+                    // lanes past the end of the slot copy nothing
+                    if (pl[cc] >= 0)
+                        __builtin_amdgcn_global_load_lds(src, (lds_ptr_t)(dst + cc * 256), 4, 0, 0);
+                }
             }
         }
     };
-    // HBM -> registers: the target values of the group in blob buf
+
+    // ---- cell waves ------------------------------------------------------
     auto load_targets = [&](int buf, T *u) {
         const TgtRec *r = recs(buf);
-        const int nt = __builtin_amdgcn_readfirstlane(hdr_nt(buf));
 #pragma unroll
         for (int i = 0; i < GMAX; ++i) {
-            if (i < nt) {
-                const int4 h0 = reinterpret_cast<const int4 *>(r + i)[0];   // comp ..
-                const int4 h1 = reinterpret_cast<const int4 *>(r + i)[1];   // Q_off, ext[3]
-                if (has_cell && g0 < h1.y && g1 < h1.z && g2 < h1.w)
-                    u[i] = a.U[(int64_t)h0.x * a.slab + gidx];
-            }
+            const int4 h0 = reinterpret_cast<const int4 *>(r + i)[0];   // comp, ...
+            const int4 h1 = reinterpret_cast<const int4 *>(r + i)[1];   // Q_off, ext[3]
+            if (has_cell && g0 < h1.y && g1 < h1.z && g2 < h1.w)
+                u[i] = a.U[(int64_t)h0.x * a.slab + gidx];
         }
     };
+    // coefficient: staged field (off >= 0), scalar (off == -1) or, in the
+    // GEN variant only, a field read from global memory (off == -2)
     auto coef = [&](int off, const void *p, double s) -> T {
-        if (off >= 0) return cfa[off + tid];
-        if (off == -1) return (T)s;
-        return in_store ? reinterpret_cast<const T *>(p)[gidx] : (T)0;
+        const T staged = cfa[(off > 0 ? off : 0) + tid];
+        T v = off >= 0 ? staged : (T)s;
+        if (GEN && off == -2) v = in_store ? reinterpret_cast<const T *>(p)[gidx] : (T)0;
+        return v;
+    };
+    // Every group holds GMAX target records; unused ones are zero records
+    // (extent 0, terms in the zero tile), so the loop has no branches and the
+    // compiler can overlap one target's LDS reads with another's arithmetic.
+    auto update = [&](int buf, const T *u_in) {
+        const TgtRec *rb = recs(buf);
+#pragma unroll
+        for (int i = 0; i < GMAX; ++i) {
+            const TgtRec &r = rb[i];
+            const int4 h0 = reinterpret_cast<const int4 *>(&r)[0];   // comp filt sig_off E_off
+            const int4 h1 = reinterpret_cast<const int4 *>(&r)[1];   // Q_off ext0 ext1 ext2
+            const bool act = has_cell && g0 < h1.y && g1 < h1.z && g2 < h1.w;
+            // W = -sumcell(...): the coefficients carry -M(j,k)/h; padding
+            // terms add coef 0 times a difference within the zero tile
+            T w = 0;
+#pragma unroll
+            for (int q = 0; q < NT; ++q) {
+                const Term tm = r.t[q];
+                const int o = tm.off + lbase;
+                w += (T)tm.coef * (lds[o + tm.stride] - lds[o]);
+            }
+            const T sig = coef(h0.z, r.sig_p, r.sig_s);
+            const T E = coef(h0.w, r.E_p, r.E_s);
+            const T wq = w + coef(h1.x, r.Q_p, 0.0);
+            T u = u_in[i];
+#pragma unroll
+            for (int app = 0; app < NAPP; ++app) {
+                // U + dt/2*(W+Q - s.*U).*E   (starmap_solver.m:352-354)
+                u = u + (half_dt * (wq - sig * u)) * E;
+                const bool s1 = a.fslot[app] != 0;
+                const double fs = h0.y ? (s1 ? r.F_s[1] : r.F_s[0]) : 1.0;
+                T f = (T)fs;
+                if (GEN && h0.y) {
+                    const void *fp = s1 ? r.F[1] : r.F[0];
+                    if (fp && act) f = reinterpret_cast<const T *>(fp)[gidx];
+                }
+                u = u * f;
+            }
+            if (act) a.U[(int64_t)h0.x * a.slab + gidx] = u;
+        }
     };
 
+    // ---- prologue ----------------------------------------------------------
     T u_cur[GMAX], u_nxt[GMAX];
-    if (ng > 0) blob_dma(0, 0);
-    if (ng > 1) blob_dma(1, 1);
-    __builtin_amdgcn_s_waitcnt(0);
+    if (copier) {
+        if (ng > 0) blob_dma(0, 0);
+        if (ng > 1) blob_dma(1, 1);
+        __builtin_amdgcn_s_waitcnt(0);
+    } else {
+        for (int c = 0; c < a.ncf; ++c)
+            cfa[c * 256 + tid] = in_store ? reinterpret_cast<const T *>(a.cfields[c])[gidx] : (T)0;
+        for (int e = tid; e < S; e += 256) lds[e] = (T)0;   // slot 0: the zero tile
+    }
     __syncthreads();
-    if (ng > 0) {
-        src_dma(0);
+    if (copier) {
+        if (ng > 0) src_dma(0);
+        __builtin_amdgcn_s_waitcnt(0);
+    } else if (ng > 0) {
         load_targets(0, u_cur);
     }
-    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
 
+    // ---- groups ------------------------------------------------------------
     for (int gq = 0; gq < ng; ++gq) {
-        const int buf = gq % 3;
-        if (gq + 1 < ng) {
-            src_dma((gq + 1) % 3);
-            load_targets((gq + 1) % 3, u_nxt);
+        if (copier) {
+            if (gq + 1 < ng) src_dma((gq + 1) % 3);
+            if (gq + 2 < ng) blob_dma(gq + 2, (gq + 2) % 3);
+            __builtin_amdgcn_s_waitcnt(0);
+        } else {
+            if (gq + 1 < ng) load_targets((gq + 1) % 3, u_nxt);
+            update(gq % 3, u_cur);
         }
-        if (gq + 2 < ng) blob_dma(gq + 2, (gq + 2) % 3);
-        const TgtRec *rb = recs(buf);
-        const int nt = __builtin_amdgcn_readfirstlane(hdr_nt(buf));
-#pragma unroll
-        for (int i = 0; i < GMAX; ++i) {
-            if (i < nt) {
-                // one batch of wide LDS reads for the whole record
-                const TgtRec r = rb[i];
-                const bool act = has_cell && g0 < r.ext[0] && g1 < r.ext[1] && g2 < r.ext[2];
-                // W = -sumcell(...): the coefficients carry -M(j,k)/h;
-                // padding terms add coef 0 times a zero difference
-                T w = 0;
-                if (__builtin_amdgcn_readfirstlane(r.nterm) > 0) {
-#pragma unroll
-                    for (int q = 0; q < NT; ++q) {
-                        const int o = r.t[q].off + lbase;
-                        w += (T)r.t[q].coef * (lds[o + r.t[q].stride] - lds[o]);
-                    }
-                }
-                const T sig = coef(r.sig_off, r.sig_p, r.sig_s);
-                const T E = coef(r.E_off, r.E_p, r.E_s);
-                const T wq = r.Q_off == -1 ? w : w + coef(r.Q_off, r.Q_p, 0.0);
-                T u = u_cur[i];
-                for (int app = 0; app < a.napp; ++app) {
-                    // U + dt/2*(W+Q - s.*U).*E   (starmap_solver.m:352-354)
-                    u = u + (half_dt * (wq - sig * u)) * E;
-                    if (r.filt) {
-                        const bool s1 = a.fslot[app] != 0;
-                        const void *fp = s1 ? r.F[1] : r.F[0];
-                        const double fs = s1 ? r.F_s[1] : r.F_s[0];
-                        u = u * (fp && act ? reinterpret_cast<const T *>(fp)[gidx] : (T)fs);
-                    }
-                }
-                if (act) a.U[(int64_t)r.comp * a.slab + gidx] = u;
-            }
-        }
-        // the copies of group g+1 (and its target values) must have landed
-        // and every wave must be done with group g before anyone goes on
-        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
+        if (!copier) {
 #pragma unroll
-        for (int i = 0; i < GMAX; ++i) u_cur[i] = u_nxt[i];
+            for (int i = 0; i < GMAX; ++i) u_cur[i] = u_nxt[i];
+        }
     }
 }
 
-template <typename T, int NT>
-static hipError_t launch_nt(const SubstepArgs<T> &a, int threads, size_t lds_bytes,
-                            hipStream_t s) {
+template <typename T, int NT, int NAPP, bool GEN>
+static hipError_t launch_v(const SubstepArgs<T> &a, int threads, size_t lds_bytes, hipStream_t s) {
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void *)substep_kernel<T, NT>,
+        hipError_t e = hipFuncSetAttribute((const void *)substep_kernel<T, NT, NAPP, GEN>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            160 * 1024);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
     const int nb = ((a.n_tiles + 7) / 8) * 8;
-    hipLaunchKernelGGL((substep_kernel<T, NT>), dim3(nb), dim3(threads), lds_bytes, s, a);
+    hipLaunchKernelGGL((substep_kernel<T, NT, NAPP, GEN>), dim3(nb), dim3(threads), lds_bytes, s,
+                       a);
     return hipGetLastError();
 }
 
+template <typename T, int NT>
+static hipError_t launch_nt(const SubstepArgs<T> &a, bool gen, int threads, size_t lds_bytes,
+                            hipStream_t s) {
+    if (a.napp == 1)
+        return gen ? launch_v<T, NT, 1, true>(a, threads, lds_bytes, s)
+                   : launch_v<T, NT, 1, false>(a, threads, lds_bytes, s);
+    return gen ? launch_v<T, NT, 2, true>(a, threads, lds_bytes, s)
+               : launch_v<T, NT, 2, false>(a, threads, lds_bytes, s);
+}
+
 template <typename T>
-hipError_t launch_substep(const SubstepArgs<T> &a, int nt, int threads, size_t lds_bytes,
-                          hipStream_t s) {
-    if (a.ncf > MAX_CFIELDS || threads != 256) return hipErrorInvalidValue;
-    if (nt <= 4) return launch_nt<T, 4>(a, threads, lds_bytes, s);
-    if (nt <= 8) return launch_nt<T, 8>(a, threads, lds_bytes, s);
-    return launch_nt<T, MAXT>(a, threads, lds_bytes, s);
+hipError_t launch_substep(const SubstepArgs<T> &a, int nt, bool gen, int threads,
+                          size_t lds_bytes, hipStream_t s) {
+    if (a.ncf > MAX_CFIELDS || threads != KTHREADS || a.chunks > CPS_MAX ||
+        (a.napp != 1 && a.napp != 2))
+        return hipErrorInvalidValue;
+    if (nt <= 8) return launch_nt<T, 8>(a, gen, threads, lds_bytes, s);
+    return launch_nt<T, MAXT>(a, gen, threads, lds_bytes, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -392,7 +459,8 @@ hipError_t launch_reduce_final(const double *part, int blocks, double *out3, hip
 }
 
 #define SMP_INSTANTIATE(T)                                                                     \
-    template hipError_t launch_substep<T>(const SubstepArgs<T> &, int, int, size_t, hipStream_t); \
+    template hipError_t launch_substep<T>(const SubstepArgs<T> &, int, bool, int, size_t,        \
+                                          hipStream_t);                                         \
     template hipError_t launch_decay<T>(const T *, T *, int64_t, double, hipStream_t);        \
     template hipError_t launch_scale<T>(T *, const T *, double, int64_t, hipStream_t);        \
     template hipError_t launch_interp<T>(const T *, const T *, double *, int64_t, double,     \

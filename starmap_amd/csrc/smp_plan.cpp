@@ -136,7 +136,7 @@ std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slo
     // groups of consecutive targets; a group's tiles are copied in while the
     // previous group is still read, so two consecutive groups must fit
     // together: each group gets half of the slots
-    const int group_cap = slot_capacity / 2;
+    const int group_cap = (slot_capacity - 1) / 2;
     std::vector<std::pair<int, int>> groups;
     std::vector<std::vector<int>> gsrc;
     for (size_t t = 0; t < tg.size();) {
@@ -159,7 +159,10 @@ std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slo
     // whose next use is farthest away
     const int INF = std::numeric_limits<int>::max();
     std::vector<int> slot_of(g.n_sys, -1);
+    // slot 0 is a zero tile: padding terms read it
     std::vector<int> owner(slot_capacity, -1);
+    if (slot_capacity < 2) return "too few LDS slots";
+    owner[0] = -2;
     auto next_use = [&](int k, size_t from) {
         for (size_t q = from; q < gsrc.size(); ++q)
             if (std::binary_search(gsrc[q].begin(), gsrc[q].end(), k)) return (int)q;
@@ -171,11 +174,11 @@ std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slo
         for (int k : gsrc[q]) {
             if (slot_of[k] >= 0) continue;
             int s = -1;
-            for (int c = 0; c < slot_capacity; ++c)
-                if (owner[c] < 0) { s = c; break; }
+            for (int c = 1; c < slot_capacity; ++c)
+                if (owner[c] == -1) { s = c; break; }
             if (s < 0) {
                 int best = -1, far = -1;
-                for (int c = 0; c < slot_capacity; ++c) {
+                for (int c = 1; c < slot_capacity; ++c) {
                     int o = owner[c];
                     if (std::binary_search(gsrc[q].begin(), gsrc[q].end(), o)) continue;
                     // the previous group still reads its sources while this
@@ -251,10 +254,10 @@ std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slo
                 }
             }
             r.nterm = nt;
-            // padding terms (coefficient 0, stride 0, reading the first real
-            // term's staged sample) keep the kernel's term loop free of branches
+            // padding terms (coefficient 0 times a difference within the zero
+            // tile of slot 0) keep the kernel's term loop free of branches
             for (int q = nt; q < MAXT; ++q) {
-                r.t[q].off = nt > 0 ? r.t[0].off : 0;
+                r.t[q].off = 0;
                 r.t[q].stride = 0;
                 r.t[q].coef = 0.0;
             }

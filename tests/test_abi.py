@@ -97,8 +97,8 @@ def test_host_only_handle_grid_types_and_schedule(clo, N, dim, monkeypatch):
         assert groups >= 1 and slots >= 1
         assert loads == sources
     s.close()
-    # default (two workgroups per CU): re-staging stays bounded
-    monkeypatch.delenv("SMP_BLOCKS_PER_CU")
+    # two workgroups per CU: re-staging stays bounded
+    monkeypatch.setenv("SMP_BLOCKS_PER_CU", "2")
     s = HipSolver(par, device=-1)
     for kind in (0, 1):
         groups, slots, loads, sources = s.plan_stats(kind)
