@@ -126,6 +126,14 @@ def main():
     bytes_per_launch = sub_bytes / launches
     mean_launch_s = sub_ms / 1e3 / launches
     achieved = bytes_per_launch / mean_launch_s / 1e9
+    # HBM bytes per launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE runs of
+    # this workload, calibrated with a copy kernel (profiles/*_traffic.json)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "substep_kernel_p%d_%dx%d_%s_traffic.json"
+                         % (args.n_mom, args.n, args.n, main_key))
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
     out = {
         "metric": METRIC,
         "value": mcu / wall,
@@ -152,7 +160,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "kernel": "substep_kernel",
             "bytes_per_launch": bytes_per_launch,
             "mean_launch_ms": mean_launch_s * 1e3,
@@ -189,6 +197,7 @@ def main():
             "rel_l2_vs_fp64_max_over_moments": worst,
             "rel_l2_vs_fp64_all_moments": glob,
             "worst_moment": int(rel.argmax()) + 1,
+            "worst_moment_norm_over_max": float(norms[rel.argmax()] / big),
             "roofline_achieved_GBs": ach32,
             "roofline_frac": ach32 / HBM_PEAK_GBS,
         }

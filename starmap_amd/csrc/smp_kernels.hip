@@ -84,7 +84,6 @@ __global__ __launch_bounds__(KTHREADS) void substep_kernel(const SubstepArgs<T> 
     const int lbase = c0 + a.lds[0] * (c1 + a.lds[1] * c2);
     const int64_t gidx = g0 + (int64_t)a.pitch0 * (g1 + (int64_t)a.ext1 * g2);
     const int S = a.slot_elems;
-    const T half_dt = (T)a.half_dt;
     const int ng = a.ngroups;
     constexpr int DPE = sizeof(T) / 4;      // dwords per element
     constexpr int EPC = 64 / DPE;           // elements per 64-lane DMA chunk
@@ -190,12 +189,16 @@ __global__ __launch_bounds__(KTHREADS) void substep_kernel(const SubstepArgs<T> 
                 u[i] = a.U[(int64_t)h0.x * a.slab + gidx];
         }
     };
+    // The update computes in FP64 whatever the state precision: the FP32
+    // state only halves the bytes moved, its flux sums and half-steps are
+    // not rounded to FP32 before the final store.
+    typedef double A;
     // coefficient: staged field (off >= 0), scalar (off == -1) or, in the
     // GEN variant only, a field read from global memory (off == -2)
-    auto coef = [&](int off, const void *p, double s) -> T {
-        const T staged = cfa[(off > 0 ? off : 0) + tid];
-        T v = off >= 0 ? staged : (T)s;
-        if (GEN && off == -2) v = in_store ? reinterpret_cast<const T *>(p)[gidx] : (T)0;
+    auto coef = [&](int off, const void *p, double s) -> A {
+        const A staged = (A)cfa[(off > 0 ? off : 0) + tid];
+        A v = off >= 0 ? staged : (A)s;
+        if (GEN && off == -2) v = in_store ? (A)reinterpret_cast<const T *>(p)[gidx] : (A)0;
         return v;
     };
     // Every group holds GMAX target records; unused ones are zero records
@@ -211,31 +214,32 @@ __global__ __launch_bounds__(KTHREADS) void substep_kernel(const SubstepArgs<T> 
             const bool act = has_cell && g0 < h1.y && g1 < h1.z && g2 < h1.w;
             // W = -sumcell(...): the coefficients carry -M(j,k)/h; padding
             // terms add coef 0 times a difference within the zero tile
-            T w = 0;
+            A w = 0;
 #pragma unroll
             for (int q = 0; q < NT; ++q) {
                 const Term tm = r.t[q];
                 const int o = tm.off + lbase;
-                w += (T)tm.coef * (lds[o + tm.stride] - lds[o]);
+                w += tm.coef * ((A)lds[o + tm.stride] - (A)lds[o]);
             }
-            const T sig = coef(h0.z, r.sig_p, r.sig_s);
-            const T E = coef(h0.w, r.E_p, r.E_s);
-            const T wq = w + coef(h1.x, r.Q_p, 0.0);
-            T u = u_in[i];
+            const A sig = coef(h0.z, r.sig_p, r.sig_s);
+            const A E = coef(h0.w, r.E_p, r.E_s);
+            const A wq = w + coef(h1.x, r.Q_p, 0.0);
+            A u = (A)u_in[i];
 #pragma unroll
             for (int app = 0; app < NAPP; ++app) {
                 // U + dt/2*(W+Q - s.*U).*E   (starmap_solver.m:352-354)
-                u = u + (half_dt * (wq - sig * u)) * E;
+                u = u + (a.half_dt * (wq - sig * u)) * E;
                 const bool s1 = a.fslot[app] != 0;
-                const double fs = h0.y ? (s1 ? r.F_s[1] : r.F_s[0]) : 1.0;
-                T f = (T)fs;
+                A f = h0.y ? (s1 ? r.F_s[1] : r.F_s[0]) : 1.0;
                 if (GEN && h0.y) {
                     const void *fp = s1 ? r.F[1] : r.F[0];
-                    if (fp && act) f = reinterpret_cast<const T *>(fp)[gidx];
+                    if (fp && act) f = (A)reinterpret_cast<const T *>(fp)[gidx];
                 }
-                u = u * f;
+                // the state after every half-step is a value of the state
+                // precision, as if stored: fused and separate launches agree
+                u = (A)(T)(u * f);
             }
-            if (act) a.U[(int64_t)h0.x * a.slab + gidx] = u;
+            if (act) a.U[(int64_t)h0.x * a.slab + gidx] = (T)u;
         }
     };
 
