@@ -13,7 +13,9 @@ Prints ONE JSON line: value = FP64 moment-cell updates/s
 (algorithmic bytes per launch / mean launch time from HIP events on the
 library's stream), the FP32-state variant with its error against the FP64
 state after the same steps, and a CPU baseline (the FP64 oracle on the host
-cores, a bounded sample of the same workload).
+cores, a bounded sample of the same workload).  A second leg ("gauss3d")
+measures the 3D P_7 (64 moments) 200^3 periodic Gaussian problem in FP64 and
+FP32 state.
 """
 import argparse
 import json
@@ -66,6 +68,69 @@ def run_gpu(prob, precision, steps, warmup):
     return s, dt, wall, (sub_ms, launches, sub_bytes)
 
 
+def fp32_error(s64, s32):
+    """Relative L2 per moment field of the FP32 state against the FP64 state
+    after the same steps (fields below 1e-8 of the largest are measured
+    against the largest), the worst field, and the error over all moments."""
+    norms, diffs = [], []
+    for j in range(s64.n_sys):
+        a = s64.get_state(j)
+        b = s32.get_state(j)
+        norms.append(np.linalg.norm(a))
+        diffs.append(np.linalg.norm(a - b))
+    norms, diffs = np.array(norms), np.array(diffs)
+    big = norms.max()
+    rel = np.where(norms > 1e-8 * big, diffs / np.maximum(norms, 1e-300), diffs / big)
+    return {
+        "rel_l2_vs_fp64_max_over_moments": float(rel.max()),
+        "rel_l2_vs_fp64_all_moments": float(np.sqrt((diffs ** 2).sum() / (norms ** 2).sum())),
+        "worst_moment": int(rel.argmax()) + 1,
+        "worst_moment_norm_over_max": float(norms[rel.argmax()] / big),
+    }
+
+
+def kernel_roofline(s, steps, cells, wall, elem):
+    """Substep-kernel bytes per launch, mean launch time and achieved GB/s,
+    plus the throughput priced with B_alg."""
+    tot_ms, sub_ms, launches, sub_bytes = s.timing()
+    n_even = int(np.sum((s.gt - 1).sum(axis=1) % 2 == 0))
+    n_odd = s.n_sys - n_even
+    bpm = b_alg_per_mcu(n_even, n_odd, s.n_sys, elem)
+    mcu = s.n_sys * cells * steps
+    ach = sub_bytes / launches / (sub_ms / 1e3 / launches) / 1e9
+    return {
+        "value": mcu / wall,
+        "ms_per_step": wall / steps * 1e3,
+        "achieved_GBs": ach,
+        "frac": ach / HBM_PEAK_GBS,
+        "bytes_per_launch": sub_bytes / launches,
+        "mean_launch_ms": sub_ms / launches,
+        "b_alg_frac": bpm * mcu / wall / 1e9 / HBM_PEAK_GBS,
+    }
+
+
+def gauss3d_leg(n, n_mom, steps, warmup):
+    """3D P_7 (64 moments) on an n^3 periodic grid over [-1,1]^3, Gaussian
+    initial zeroth moment (std 0.05), sigma_a=0.1, sigma_s0=1,
+    sigma_sm=0.5^m, dt = 0.5*dx: FP64 and FP32 state."""
+    from starmap_amd import problems as P
+    prob = P.gauss_periodic_3d(n=n, n_mom=n_mom, tfinal=0.75)
+    cells = n ** 3
+    leg = {"workload": "3D P%d gaussian %d^3 periodic" % (n_mom, n), "steps": steps}
+    ss = {}
+    for prec, elem in (("fp64", 8), ("fp32", 4)):
+        log("3d %s: setting up %d^3 P%d" % (prec, n, n_mom))
+        s, dt, wall, _ = run_gpu(prob, prec, steps, warmup)
+        leg[prec] = kernel_roofline(s, steps, cells, wall, elem)
+        log("3d %s: %d steps in %.3f s" % (prec, steps, wall))
+        ss[prec] = s
+    leg["fp32"]["speedup_vs_fp64"] = leg["fp32"]["value"] / leg["fp64"]["value"]
+    leg["fp32"].update(fp32_error(ss["fp64"], ss["fp32"]))
+    for s in ss.values():
+        s.close()
+    return leg
+
+
 def cpu_baseline(prob, steps):
     """The FP64 oracle (C restatement of the reference step, OpenMP over
     components) timed on the host: `steps` full steps of the same workload."""
@@ -91,6 +156,9 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32", "both"], default="both")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--c3-steps", type=int, default=50,
+                    help="steps of the 3D P7 periodic leg (0 skips it)")
+    ap.add_argument("--c3-n", type=int, default=200, help="grid side of the 3D leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the time-stepping core is a single-GPU "
@@ -174,35 +242,19 @@ def main():
     }
     if "fp32" in res and "fp64" in res:
         s32, wall32, (sub32, l32, by32) = res["fp32"]
-        bpm32 = b_alg_per_mcu(n_even, n_odd, n_sys, 4)
         ach32 = by32 / (sub32 / 1e3) / 1e9
-        # relative L2 per moment field of the FP32 state against the FP64
-        # state after the same steps
-        worst, big = 0.0, 0.0
-        norms, diffs = [], []
-        for j in range(n_sys):
-            a = s64.get_state(j)
-            b = s32.get_state(j)
-            norms.append(np.linalg.norm(a))
-            diffs.append(np.linalg.norm(a - b))
-        norms, diffs = np.array(norms), np.array(diffs)
-        big = norms.max()
-        rel = np.where(norms > 1e-8 * big, diffs / np.maximum(norms, 1e-300), diffs / big)
-        worst = float(rel.max())
-        glob = float(np.sqrt((diffs ** 2).sum() / (norms ** 2).sum()))
-        out["fp32"] = {
+        err = fp32_error(s64, s32)
+        out["fp32"] = dict({
             "value": mcu / wall32,
             "ms_per_step": wall32 / args.steps * 1e3,
             "speedup_vs_fp64": (mcu / wall32) / out["value"],
-            "rel_l2_vs_fp64_max_over_moments": worst,
-            "rel_l2_vs_fp64_all_moments": glob,
-            "worst_moment": int(rel.argmax()) + 1,
-            "worst_moment_norm_over_max": float(norms[rel.argmax()] / big),
             "roofline_achieved_GBs": ach32,
             "roofline_frac": ach32 / HBM_PEAK_GBS,
-        }
+        }, **err)
     for v in res.values():
         v[0].close()
+    if args.c3_steps > 0:
+        out["gauss3d"] = gauss3d_leg(args.c3_n, 7, args.c3_steps, min(args.warmup, 5))
 
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))

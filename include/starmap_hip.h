@@ -75,6 +75,13 @@ typedef struct smp_desc {
                                  handle (validation, grid types and schedules
                                  without touching a GPU; device calls fail
                                  with SMP_E_STATE)                             */
+    int32_t tile[3];          /* cells per workgroup tile per kernel axis (the
+                                 axes that carry cells, fastest first); zeros
+                                 take the library default.  tile[0]+1 must be
+                                 a power of two, the tile at most 256 cells   */
+    int32_t blocks_per_cu;    /* LDS budget: workgroups per CU the schedule is
+                                 made for (1..8); 0 takes SMP_BLOCKS_PER_CU
+                                 from the environment, else 1                 */
 } smp_desc;
 
 /* Last error message of the calling thread ("" if none). */

@@ -47,7 +47,9 @@ class Desc(ctypes.Structure):
                 ("f_order", ctypes.c_void_p),
                 ("f_position", ctypes.c_int32),
                 ("precision", ctypes.c_int32),
-                ("device", ctypes.c_int32)]
+                ("device", ctypes.c_int32),
+                ("tile", ctypes.c_int32 * 3),
+                ("blocks_per_cu", ctypes.c_int32)]
 
 
 _lib = None

@@ -100,6 +100,7 @@ struct smp_problem {
     void *interp_buf = nullptr;
     int64_t interp_cap = 0;
     void *red_buf = nullptr;
+    void *sink = nullptr;         // 256 elements: stores of lanes without a target cell
     std::vector<hipEvent_t> events;
     double last_total_ms = 0, last_substep_ms = 0, last_bytes = 0;
     int64_t last_launches = 0;
@@ -224,7 +225,8 @@ namespace {
 // per workgroup trade source re-staging (mostly L2 / Infinity Cache hits) for
 // occupancy; when the slots are too few for the coupling the schedule falls
 // back to fewer workgroups.  Within the chosen budget the schedule shrinks to
-// the smallest slot count that stages every source once, when that fits.
+// the smallest slot count that keeps the group count and stages every source
+// once, when that fits.
 int make_plan(smp_problem *h, int kind, int ncf_reserve) {
     std::string err = "no schedule";
     const size_t slot_bytes = (size_t)h->ts.slot_elems * h->elem;
@@ -242,13 +244,17 @@ int make_plan(smp_problem *h, int kind, int ncf_reserve) {
     }
     if (!err.empty()) return fail(SMP_E_INVALID, err);
     if ((int)plan.loads.size() == plan.n_sources) {
+        // fewest slots with the same group count (every group is one barrier
+        // phase) and no re-staging
+        const size_t ngroups = plan.groups.size();
         int lo = 2, hi = cap;
         while (lo < hi) {
             const int mid = (lo + hi) / 2;
             smp::Plan trial;
             std::string e2 = smp::build_plan(h->geo, h->ts, kind, mid, smp::GMAX, max_load_elems,
                                              smp::MAXL, trial);
-            if (e2.empty() && (int)trial.loads.size() == trial.n_sources)
+            if (e2.empty() && (int)trial.loads.size() == trial.n_sources &&
+                trial.groups.size() <= ngroups)
                 hi = mid;
             else
                 lo = mid + 1;
@@ -333,9 +339,18 @@ int smp_create(const smp_desc *d, smp_problem **out) {
     h->pitch0 = h->geo.ndim > 1 ? ((h->E[0] + 15) / 16) * 16 : h->E[0];
     h->slab = (int64_t)h->pitch0 * h->E[1] * h->E[2];
     h->slab = ((h->slab + 63) / 64) * 64;
-    h->ts = smp::choose_tile(h->geo, h->elem);
-    if (h->ts.lds[0] > 1024 || h->ts.lds[1] > 128 || h->ts.lds[2] > 128 || h->ts.threads != 256)
-        return fail(SMP_E_INVALID, "unsupported tile shape");
+    // source offsets are 32-bit dword offsets within a component's slab
+    if (h->slab * (h->elem / 4) >= ((int64_t)1 << 31))
+        return fail(SMP_E_INVALID, "a component holds more than 2^31 dwords");
+    if (d->tile[0] || d->tile[1] || d->tile[2]) {
+        err = smp::make_tile(h->geo, std::max(1, d->tile[0]), std::max(1, d->tile[1]),
+                             std::max(1, d->tile[2]), h->ts);
+        if (!err.empty()) return fail(SMP_E_INVALID, err);
+    } else {
+        h->ts = smp::choose_tile(h->geo, h->elem);
+    }
+    if ((h->ts.slot_elems * (h->elem / 4) + 63) / 64 > smp::CPS_MAX)
+        return fail(SMP_E_INVALID, "a tile with its halo exceeds 4 KB per component");
     h->n_tiles = 1;
     for (int e = 0; e < 3; ++e) {
         h->ntile[e] = (h->E[e] + h->ts.tile[e] - 1) / h->ts.tile[e];
@@ -345,7 +360,8 @@ int smp_create(const smp_desc *d, smp_problem **out) {
     // fields are known)
     int want_blocks = 1;
     if (const char *env = std::getenv("SMP_BLOCKS_PER_CU")) want_blocks = std::atoi(env);
-    if (want_blocks < 1 || want_blocks > 8) return fail(SMP_E_INVALID, "SMP_BLOCKS_PER_CU must be 1..8");
+    if (d->blocks_per_cu) want_blocks = d->blocks_per_cu;
+    if (want_blocks < 1 || want_blocks > 8) return fail(SMP_E_INVALID, "blocks_per_cu must be 1..8");
     h->blocks_per_cu = want_blocks;
     for (int kind = 0; kind < 2; ++kind) {
         int rc = make_plan(h.get(), kind, 0);
@@ -360,6 +376,8 @@ int smp_create(const smp_desc *d, smp_problem **out) {
     if (rc) return rc;
     HIP_TRY(hipMemsetAsync(h->U, 0, (size_t)h->slab * h->elem * d->n_sys, h->stream));
     rc = h->dalloc(&h->red_buf, sizeof(double) * (3 * 1024 + 3));
+    if (rc) return rc;
+    rc = h->dalloc(&h->sink, sizeof(double) * 256);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(h->stream));
     *out = h.release();
@@ -637,6 +655,18 @@ int refresh_coefs(smp_problem *h) {
             r.F_s[1] = c.F_s[1];
         }
         h->ncf[kind] = (int)table.size();
+        // device form of the records: byte offsets from the start of the
+        // kernel's LDS (blob ring, staged coefficient fields, source slots)
+        // and of each component in the state
+        const int32_t slots_base =
+            (int32_t)(smp::substep_lds_overhead() + (size_t)h->ncf[kind] * 256 * h->elem);
+        for (smp::TgtRec &r : recs) {
+            r.uoff = (int64_t)r.comp * h->slab * h->elem;
+            for (int q = 0; q < smp::MAXT; ++q) {
+                r.t[q].off = slots_base + r.t[q].off * h->elem;
+                r.t[q].stride *= h->elem;
+            }
+        }
         h->gen[kind] = false;
         for (const smp::TgtRec &r : recs)
             if (r.sig_off == -2 || r.E_off == -2 || r.Q_off == -2 || r.F[0] || r.F[1]) h->gen[kind] = true;
@@ -656,6 +686,13 @@ int refresh_coefs(smp_problem *h) {
             std::memcpy(b, hdr, sizeof(hdr));
             std::memcpy(b + smp::BLOB_REC_OFF, recs.data() + gr.t0,
                         sizeof(smp::TgtRec) * (gr.t1 - gr.t0));
+            // unused records: no cells (extent 0), every term in the zero tile
+            for (int i = gr.t1 - gr.t0; i < smp::GMAX; ++i) {
+                smp::TgtRec z;
+                std::memset(&z, 0, sizeof(z));
+                for (int q = 0; q < smp::MAXT; ++q) z.t[q].off = slots_base;
+                std::memcpy(b + smp::BLOB_REC_OFF + i * sizeof(smp::TgtRec), &z, sizeof(z));
+            }
             std::memcpy(b + smp::BLOB_LD_OFF, p.loads.data() + gr.l0,
                         sizeof(smp::DevLoad) * (gr.l1 - gr.l0));
         }
@@ -670,6 +707,7 @@ smp::SubstepArgs<T> make_args(smp_problem *h, int kind, double dt, int napp, int
     smp::SubstepArgs<T> a;
     std::memset(&a, 0, sizeof(a));
     a.U = (T *)h->U;
+    a.sink = (T *)h->sink;
     a.slab = h->slab;
     a.pitch0 = h->pitch0;
     a.ext1 = h->E[1];
@@ -686,9 +724,6 @@ smp::SubstepArgs<T> make_args(smp_problem *h, int kind, double dt, int napp, int
         a.lds[e] = h->ts.lds[e];
     }
     a.slot_elems = h->ts.slot_elems;
-    a.lds0_shift = 0;
-    while ((1 << a.lds0_shift) < h->ts.lds[0]) ++a.lds0_shift;
-    a.inv_lds1 = 1.0f / (float)h->ts.lds[1];
     a.chunks = (h->ts.slot_elems * (h->elem / 4) + 63) / 64;
     a.n_tiles = h->n_tiles;
     const smp::Plan &p = h->plan[kind];

@@ -8,8 +8,7 @@
 namespace smp {
 
 constexpr int GMAX = 8;       // targets per group
-constexpr int PF = 16;        // staged source elements per thread per group
-constexpr int MAXL = 12;      // source loads per group
+constexpr int MAXL = 19;      // source loads per group (as many as a blob holds)
 constexpr int MAX_GROUPS = 256;
 constexpr int MAX_CFIELDS = 12; // coefficient fields staged per tile
 
@@ -24,13 +23,12 @@ constexpr int KTHREADS = 384; // 4 cell waves + 2 copy waves
 template <typename T>
 struct SubstepArgs {
     T *U;
+    T *sink;                  // 256 elements written by lanes without a target cell
     int64_t slab;             // elements per component
     int32_t pitch0, ext1;     // storage index = i0 + pitch0*(i1 + ext1*i2)
     int32_t n[3], bc[3];      // per kernel axis
     int32_t tile[3], ntile[3], lds[3];
-    int32_t slot_elems;       // lds[0]*lds[1]*lds[2]; lds[0] is a power of two
-    int32_t lds0_shift;       // log2(lds[0])
-    float inv_lds1;           // 1/lds[1]
+    int32_t slot_elems;       // lds[0]*lds[1]*lds[2]
     int32_t chunks;           // DMA chunks (64 lanes x 4 bytes) per slot
     int32_t n_tiles;          // total tiles
     int32_t ngroups;

@@ -98,28 +98,46 @@ __global__ __launch_bounds__(KTHREADS) void substep_kernel(const SubstepArgs<T> 
     };
 
     // ---- copy wave -------------------------------------------------------
-    // tile-local coordinates and, for tiles away from the domain boundary,
-    // the slab offset of the element each lane copies in chunk cc
-    bool interior = true;
-    for (int e = 0; e < 3; ++e) {
-        const int o = e == 0 ? o0 : e == 1 ? o1 : o2;
-        if (a.lds[e] > 1 && (o < 1 || o + a.lds[e] > a.n[e])) interior = false;
-    }
-    const int64_t tile_base = o0 + (int64_t)a.pitch0 * (o1 + (int64_t)a.ext1 * o2);
-    int pl[CPS_MAX];   // packed l0 | l1 << 10 | l2 << 17 of the lane's element, -1 past the slot
+    // Per tile, the dword offset in a source slab of the element each lane
+    // copies in chunk cc for a face-centred source (no origin shift), with
+    // boundary wrapping / clamping resolved (-1 past the slot), and per axis
+    // a bit per chunk for lanes at coordinate 0.  A cell-centred source looks
+    // one cell lower on its shifted axes: its coordinate is x-1, or at x == 0
+    // n-1 (periodic) / 0 (extrapolation, clamped).  Where no lane is at
+    // x == 0, a uniform shift of the slab base does it.
+    int po[CPS_MAX];
+    int zb[3] = {0, 0, 0};
+    const int st[3] = {DPE, a.pitch0 * DPE, a.pitch0 * a.ext1 * DPE};
+    int wd[3];           // offset change at x == 0 for a shifted axis
+#pragma unroll
+    for (int e = 0; e < 3; ++e) wd[e] = a.bc[e] == 0 ? (a.n[e] - 1) * st[e] : 0;
+    bool low_inner = false;
     if (copier) {
 #pragma unroll
         for (int cc = 0; cc < CPS_MAX; ++cc) {
             const int e = cc * EPC + lane / DPE;
-            pl[cc] = -1;
+            po[cc] = -1;
             if (cc < a.chunks && e < S) {
-                const int l0 = e & (a.lds[0] - 1);
-                const int row = e >> a.lds0_shift;
-                const int l2 = (int)(((float)row + 0.5f) * a.inv_lds1);
-                const int l1 = row - l2 * a.lds[1];
-                pl[cc] = l0 | (l1 << 10) | (l2 << 17);
+                const int l0 = e % a.lds[0];
+                const int row = e / a.lds[0];
+                const int l1 = row % a.lds[1];
+                const int l2 = row / a.lds[1];
+                const int x0 = wrap_or_clamp(o0 + l0, a.n[0], a.bc[0], a.n[0] + a.bc[0]);
+                const int x1 = wrap_or_clamp(o1 + l1, a.n[1], a.bc[1], a.n[1] + a.bc[1]);
+                const int x2 = wrap_or_clamp(o2 + l2, a.n[2], a.bc[2], a.n[2] + a.bc[2]);
+                po[cc] = (x0 + a.pitch0 * (x1 + a.ext1 * x2)) * DPE + (DPE == 2 ? (lane & 1) : 0);
+                zb[0] |= (x0 == 0) << cc;
+                zb[1] |= (x1 == 0) << cc;
+                zb[2] |= (x2 == 0) << cc;
             }
         }
+        // x == 0 occurs at the low boundary and, periodic, where the tile
+        // wraps at the top; a wave without such a lane shifts uniformly
+        int zany = 0;
+#pragma unroll
+        for (int e = 0; e < 3; ++e)
+            if (a.lds[e] > 1) zany |= zb[e];
+        low_inner = !__any(zany != 0);
     }
     auto blob_dma = [&](int gq, int buf) {
         // three 1 KB chunks, 16 bytes a lane, by the first copy wave
@@ -147,48 +165,52 @@ __global__ __launch_bounds__(KTHREADS) void substep_kernel(const SubstepArgs<T> 
             const int rec = __builtin_amdgcn_readlane(myrec, li);
             const int comp = rec & 0xFFFFF;
             const int slot = (rec >> 20) & 0xFF;
-            const int m = (rec >> 28) & 7;   // bit e: -1 origin shift (cell-centred source)
-            const int s0 = -(m & 1), s1 = -((m >> 1) & 1), s2 = -((m >> 2) & 1);
-            const T *cb = a.U + (int64_t)comp * a.slab;
+            const int m = (rec >> 28) & 7;   // bit e: cell-centred on kernel axis e
+            const unsigned *cb = reinterpret_cast<const unsigned *>(a.U + (int64_t)comp * a.slab);
             unsigned char *dst = reinterpret_cast<unsigned char *>(lds + slot * S);
+            if (low_inner) {
+                // every shifted coordinate is x-1: one uniform base shift
+                const unsigned *cbs = cb - ((m & 1) ? st[0] : 0) - ((m & 2) ? st[1] : 0) -
+                                      ((m & 4) ? st[2] : 0);
 #pragma unroll
-            for (int cc = 0; cc < CPS_MAX; ++cc) {
-                if (cc < a.chunks) {
-                    const int l0 = pl[cc] & 1023, l1 = (pl[cc] >> 10) & 127, l2 = (pl[cc] >> 17) & 127;
-                    int64_t off;
-                    if (interior) {
-                        off = tile_base + (l0 + s0) +
-                              (int64_t)a.pitch0 * ((l1 + s1) + (int64_t)a.ext1 * (l2 + s2));
-                    } else {
-                        const int x0 = wrap_or_clamp(o0 + l0 + s0, a.n[0], a.bc[0],
-                                                     s0 ? a.n[0] : a.n[0] + a.bc[0]);
-                        const int x1 = wrap_or_clamp(o1 + l1 + s1, a.n[1], a.bc[1],
-                                                     s1 ? a.n[1] : a.n[1] + a.bc[1]);
-                        const int x2 = wrap_or_clamp(o2 + l2 + s2, a.n[2], a.bc[2],
-                                                     s2 ? a.n[2] : a.n[2] + a.bc[2]);
-                        off = x0 + (int64_t)a.pitch0 * (x1 + (int64_t)a.ext1 * x2);
-                    }
-                    const unsigned *src = reinterpret_cast<const unsigned *>(cb + off) +
-                                          (DPE == 2 ? (lane & 1) : 0);
+                for (int cc = 0; cc < CPS_MAX; ++cc) {
                     // lanes past the end of the slot copy nothing
-                    if (pl[cc] >= 0)
-                        __builtin_amdgcn_global_load_lds(src, (lds_ptr_t)(dst + cc * 256), 4, 0, 0);
+                    if (cc < a.chunks && po[cc] >= 0)
+                        __builtin_amdgcn_global_load_lds(cbs + po[cc], (lds_ptr_t)(dst + cc * 256),
+                                                         4, 0, 0);
+                }
+            } else {
+#pragma unroll
+                for (int cc = 0; cc < CPS_MAX; ++cc) {
+                    if (cc < a.chunks && po[cc] >= 0) {
+                        int off = po[cc];
+#pragma unroll
+                        for (int e = 0; e < 3; ++e)
+                            if (m & (1 << e)) off += ((zb[e] >> cc) & 1) ? wd[e] : -st[e];
+                        __builtin_amdgcn_global_load_lds(cb + off, (lds_ptr_t)(dst + cc * 256), 4, 0,
+                                                         0);
+                    }
                 }
             }
         }
     };
 
     // ---- cell waves ------------------------------------------------------
+    // Target loads and stores are issued by every lane (lanes outside a
+    // target's extent read element 0 and write to the sink), so that the
+    // number of memory operations per group is fixed and the compiler can
+    // wait for a prefetched target value with an exact count instead of
+    // draining every load and store in flight.
+    const char *ug_ld = reinterpret_cast<const char *>(a.U) + (in_store ? gidx : 0) * sizeof(T);
+    char *ug_st = reinterpret_cast<char *>(a.U) + gidx * sizeof(T);
     auto load_targets = [&](int buf, T *u) {
         const TgtRec *r = recs(buf);
 #pragma unroll
-        for (int i = 0; i < GMAX; ++i) {
-            const int4 h0 = reinterpret_cast<const int4 *>(r + i)[0];   // comp, ...
-            const int4 h1 = reinterpret_cast<const int4 *>(r + i)[1];   // Q_off, ext[3]
-            if (has_cell && g0 < h1.y && g1 < h1.z && g2 < h1.w)
-                u[i] = a.U[(int64_t)h0.x * a.slab + gidx];
-        }
+        for (int i = 0; i < GMAX; ++i) u[i] = *reinterpret_cast<const T *>(ug_ld + r[i].uoff);
     };
+    // byte address of this thread's cell in the first source slot; term
+    // offsets are byte offsets from the start of LDS
+    const int lb = lbase * (int)sizeof(T);
     // The update computes in FP64 whatever the state precision: the FP32
     // state only halves the bytes moved, its flux sums and half-steps are
     // not rounded to FP32 before the final store.
@@ -211,15 +233,16 @@ __global__ __launch_bounds__(KTHREADS) void substep_kernel(const SubstepArgs<T> 
             const TgtRec &r = rb[i];
             const int4 h0 = reinterpret_cast<const int4 *>(&r)[0];   // comp filt sig_off E_off
             const int4 h1 = reinterpret_cast<const int4 *>(&r)[1];   // Q_off ext0 ext1 ext2
-            const bool act = has_cell && g0 < h1.y && g1 < h1.z && g2 < h1.w;
+            const bool act = has_cell & (g0 < h1.y) & (g1 < h1.z) & (g2 < h1.w);
             // W = -sumcell(...): the coefficients carry -M(j,k)/h; padding
             // terms add coef 0 times a difference within the zero tile
             A w = 0;
 #pragma unroll
             for (int q = 0; q < NT; ++q) {
                 const Term tm = r.t[q];
-                const int o = tm.off + lbase;
-                w += tm.coef * ((A)lds[o + tm.stride] - (A)lds[o]);
+                const unsigned char *lo = smem_raw + (tm.off + lb);
+                w += tm.coef * ((A)*reinterpret_cast<const T *>(lo + tm.stride) -
+                                (A)*reinterpret_cast<const T *>(lo));
             }
             const A sig = coef(h0.z, r.sig_p, r.sig_s);
             const A E = coef(h0.w, r.E_p, r.E_s);
@@ -230,7 +253,7 @@ __global__ __launch_bounds__(KTHREADS) void substep_kernel(const SubstepArgs<T> 
                 // U + dt/2*(W+Q - s.*U).*E   (starmap_solver.m:352-354)
                 u = u + (a.half_dt * (wq - sig * u)) * E;
                 const bool s1 = a.fslot[app] != 0;
-                A f = h0.y ? (s1 ? r.F_s[1] : r.F_s[0]) : 1.0;
+                A f = s1 ? r.F_s[1] : r.F_s[0];   // 1 for targets without a filter
                 if (GEN && h0.y) {
                     const void *fp = s1 ? r.F[1] : r.F[0];
                     if (fp && act) f = (A)reinterpret_cast<const T *>(fp)[gidx];
@@ -239,12 +262,13 @@ __global__ __launch_bounds__(KTHREADS) void substep_kernel(const SubstepArgs<T> 
                 // precision, as if stored: fused and separate launches agree
                 u = (A)(T)(u * f);
             }
-            if (act) a.U[(int64_t)h0.x * a.slab + gidx] = (T)u;
+            T *dst = act ? reinterpret_cast<T *>(ug_st + r.uoff) : a.sink + tid;
+            *dst = (T)u;
         }
     };
 
     // ---- prologue ----------------------------------------------------------
-    T u_cur[GMAX], u_nxt[GMAX];
+    T uA[GMAX], uB[GMAX];
     if (copier) {
         if (ng > 0) blob_dma(0, 0);
         if (ng > 1) blob_dma(1, 1);
@@ -259,25 +283,29 @@ __global__ __launch_bounds__(KTHREADS) void substep_kernel(const SubstepArgs<T> 
         if (ng > 0) src_dma(0);
         __builtin_amdgcn_s_waitcnt(0);
     } else if (ng > 0) {
-        load_targets(0, u_cur);
+        load_targets(0, uA);
     }
     __syncthreads();
 
     // ---- groups ------------------------------------------------------------
+    // one barrier phase per group; the cell waves load the target values of
+    // the next group after storing this group's, into the other of two
+    // register sets (no register copies between phases): the loads complete
+    // while the copy waves finish, and the next phase's first read of them is
+    // the only wait for memory in the cell waves
     for (int gq = 0; gq < ng; ++gq) {
         if (copier) {
             if (gq + 1 < ng) src_dma((gq + 1) % 3);
             if (gq + 2 < ng) blob_dma(gq + 2, (gq + 2) % 3);
             __builtin_amdgcn_s_waitcnt(0);
+        } else if (gq & 1) {
+            update(gq % 3, uB);
+            if (gq + 1 < ng) load_targets((gq + 1) % 3, uA);
         } else {
-            if (gq + 1 < ng) load_targets((gq + 1) % 3, u_nxt);
-            update(gq % 3, u_cur);
+            update(gq % 3, uA);
+            if (gq + 1 < ng) load_targets((gq + 1) % 3, uB);
         }
         __syncthreads();
-        if (!copier) {
-#pragma unroll
-            for (int i = 0; i < GMAX; ++i) u_cur[i] = u_nxt[i];
-        }
     }
 }
 

@@ -85,23 +85,34 @@ std::string build_geometry(Geometry &g) {
     return "";
 }
 
-TileShape make_tile(const Geometry &g, int tx, int ty, int tz) {
-    TileShape ts;
-    int t[3] = {tx, ty, tz};
-    for (int e = 0; e < 3; ++e) ts.tile[e] = e < g.ndim ? t[e] : 1;
+std::string make_tile(const Geometry &g, int tx, int ty, int tz, TileShape &ts) {
+    ts = TileShape();
+    const int t[3] = {tx, ty, tz};
+    for (int e = 0; e < 3; ++e) {
+        ts.tile[e] = e < g.ndim ? t[e] : 1;
+        if (ts.tile[e] < 1) return "tile extents must be positive";
+    }
+    if (ts.cells() > 256) return "a tile holds at most 256 cells";
+    // LDS rows of a power of two: the tile plus its one-cell halo
+    if (g.ndim > 1 && ((ts.tile[0] + 1) & ts.tile[0])) return "tile[0]+1 must be a power of two";
     ts.threads = 256;
     for (int e = 0; e < 3; ++e) ts.lds[e] = e < g.ndim ? ts.tile[e] + 1 : 1;
     ts.slot_elems = ts.lds[0] * ts.lds[1] * ts.lds[2];
-    return ts;
+    return "";
 }
 
 TileShape choose_tile(const Geometry &g, int elem_bytes) {
     (void)elem_bytes;
     // the first axis spans a power of two with its halo (LDS rows of 256, 64
     // or 32 elements); 256 threads, one per cell, spare threads only copy
-    if (g.ndim == 1) return make_tile(g, 255, 1, 1);
-    if (g.ndim == 2) return make_tile(g, 63, 4, 1);
-    return make_tile(g, 31, 4, 2);
+    TileShape ts;
+    if (g.ndim <= 1)
+        make_tile(g, 255, 1, 1, ts);
+    else if (g.ndim == 2)
+        make_tile(g, 63, 4, 1, ts);
+    else
+        make_tile(g, 31, 4, 2, ts);
+    return ts;
 }
 
 std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slot_capacity,
@@ -196,18 +207,11 @@ std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slo
             slot_of[k] = s;
             used = std::max(used, s + 1);
             DevLoad ld;
+            std::memset(&ld, 0, sizeof(ld));
             ld.comp = k;
             ld.slot = s;
-            for (int e = 0; e < 3; ++e) {
-                if (e < g.ndim) {
-                    int ax = g.axis[e];
-                    ld.org[e] = g.gt[3 * k + ax] == 1 ? -1 : 0;
-                    ld.ext[e] = g.ext_of(k, ax);
-                } else {
-                    ld.org[e] = 0;
-                    ld.ext[e] = 1;
-                }
-            }
+            for (int e = 0; e < 3; ++e)
+                ld.org[e] = e < g.ndim && g.gt[3 * k + g.axis[e]] == 1 ? -1 : 0;
             new_loads.push_back(ld);
         }
         // staging is bounded by the prefetch registers: leading load-only
@@ -253,7 +257,6 @@ std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slo
                     ++nt;
                 }
             }
-            r.nterm = nt;
             // padding terms (coefficient 0 times a difference within the zero
             // tile of slot 0) keep the kernel's term loop free of branches
             for (int q = nt; q < MAXT; ++q) {

@@ -27,13 +27,15 @@ struct DevLoad {
     int32_t comp;       // source component
     int32_t slot;       // LDS slot
     int32_t org[3];     // tile origin shift per kernel axis (0 or -1)
-    int32_t ext[3];     // source extent per kernel axis (for clamping)
+    int32_t pad[3];
 };
 
-// One coupling term: W += coef * (lds[off + lbase + stride] - lds[off + lbase])
+// One coupling term: W += coef * (lds[off + lbase + stride] - lds[off + lbase]).
+// The schedule holds element offsets relative to the first source slot; the
+// device copy holds byte offsets from the start of the kernel's LDS.
 struct alignas(16) Term {
-    int32_t off;        // LDS element offset of the low sample (slot base + shifts)
-    int32_t stride;     // LDS stride of the coupled axis
+    int32_t off;        // offset of the low sample (slot base + shifts)
+    int32_t stride;     // stride of the coupled axis
     double coef;        // -M(j,k)/h; 0 for padding terms
 };
 
@@ -49,10 +51,10 @@ struct alignas(16) TgtRec {
     int32_t Q_off;
     int32_t ext[3];     // target extent per kernel axis
     const void *sig_p, *E_p, *Q_p;
-    int32_t nterm, pad;
+    int64_t uoff;       // byte offset of the component in the state (device copy)
     const void *F[2];   // filter fields of slots k=1,2 (or null: F_s)
     double sig_s, E_s;
-    double F_s[2];
+    double F_s[2];      // filter factors of slots k=1,2 (1 without a filter)
     Term t[MAXT];
 };
 
@@ -90,8 +92,8 @@ struct TileShape {
     int cells() const { return tile[0] * tile[1] * tile[2]; }
 };
 
-// tile of tx*ty*tz <= 256 cells (256 threads, one cell each)
-TileShape make_tile(const Geometry &g, int tx, int ty, int tz);
+// tile of tx*ty*tz <= 256 cells (256 threads, one cell each); "" or an error
+std::string make_tile(const Geometry &g, int tx, int ty, int tz, TileShape &ts);
 TileShape choose_tile(const Geometry &g, int elem_bytes);
 
 struct Plan {

@@ -58,7 +58,10 @@ def _as_coef(v, shape):
 class HipSolver:
     """One StaRMAP problem on one GPU (a ``smp_problem`` handle)."""
 
-    def __init__(self, par, precision="fp64", device=0):
+    def __init__(self, par, precision="fp64", device=0, tile=None, blocks_per_cu=0):
+        """tile: cells per workgroup tile per kernel axis (None: library
+        default); blocks_per_cu: workgroups per CU the LDS schedule is made
+        for (0: library default)."""
         L = _lib.lib()
         par = dict(par)
         Mx, My, Mz = (np.asarray(par[k], dtype=np.float64) for k in ("Mx", "My", "Mz"))
@@ -120,6 +123,10 @@ class HipSolver:
         d.f_position = fpos_code
         d.precision = {"fp64": _lib.SMP_FP64, "fp32": _lib.SMP_FP32}[precision]
         d.device = int(device)
+        if tile is not None:
+            for e, v in enumerate(tile):
+                d.tile[e] = int(v)
+        d.blocks_per_cu = int(blocks_per_cu)
         self.precision = precision
         h = ctypes.c_void_p()
         _lib.check(L.smp_create(ctypes.byref(d), ctypes.byref(h)))
