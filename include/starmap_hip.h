@@ -54,6 +54,7 @@ extern "C" {
 #define SMP_COEF_FILTER   3   /* filterTerm{gt,f_order,slot}                      */
 
 typedef struct smp_problem smp_problem;
+typedef struct smp_batch smp_batch;
 
 typedef struct smp_desc {
     int32_t n_sys;            /* number of moment components, length(par.Mz)    */
@@ -162,6 +163,52 @@ int64_t smp_device_bytes(const smp_problem *h);
  * loads per tile and distinct sources. */
 int smp_plan_stats(const smp_problem *h, int32_t kind, int32_t *n_groups, int32_t *n_slots,
                    int32_t *n_loads, int32_t *n_sources);
+
+/* ---- schedule (no counterpart in the reference) ------------------------ */
+
+/* Re-make the substep schedules of the handle for another tile shape (cells
+ * per workgroup per kernel axis; NULL or zeros: library default) and LDS
+ * budget (workgroups per CU, 1..8; 0 keeps the current one).  The state and
+ * coefficients are kept; on failure the previous schedule stays in force.
+ * Used by the autotuner. */
+int smp_set_schedule(smp_problem *h, const int32_t *tile, int32_t blocks_per_cu);
+int smp_get_schedule(const smp_problem *h, int32_t *tile, int32_t *blocks_per_cu);
+
+/* ---- state checkpoints in device memory (no counterpart in the reference) */
+
+/* Copy the whole state into device checkpoint slot `slot` (0..4096,
+ * allocated on first use, kept until smp_checkpoint_free or smp_destroy), or
+ * back.  Used for recomputation between checkpoints and by the autotuner.
+ * smp_checkpoint_free releases one slot, or every slot for slot = -1. */
+int smp_checkpoint_save(smp_problem *h, int32_t slot);
+int smp_checkpoint_load(smp_problem *h, int32_t slot);
+int smp_checkpoint_free(smp_problem *h, int32_t slot);
+
+/* ---- batched stepping (no counterpart in the reference) ----------------- */
+
+/* Group n handles (same device and precision; no filter or the substep
+ * filter) for batched stepping.  The handles stay owned by the caller and
+ * must outlive the batch; each keeps its own state, coefficients and
+ * schedule. */
+int smp_batch_create(smp_problem *const *handles, int32_t n, smp_batch **out);
+int smp_batch_destroy(smp_batch *b);
+
+/* Advance problem i by n_steps[i] time steps of size dt[i] -- the same
+ * arithmetic as smp_step(handles[i], dt[i], n_steps[i]), bit for bit -- with
+ * all problems in lockstep: one persistent launch per substep position runs
+ * the substeps of every problem that has steps left, its workgroups taking
+ * (problem, tile) units from a counter, heaviest problems first.
+ * use_graph != 0 captures the launches in a hipGraph and replays it; the
+ * graph is kept and replayed by later calls with the same launch structure. */
+int smp_batch_run(smp_batch *b, const double *dt, const int64_t *n_steps, int32_t use_graph);
+
+/* Timing of the last smp_batch_run: elapsed milliseconds (HIP events on the
+ * batch's stream), the sum of the per-launch times (equal to the elapsed time
+ * when a graph ran), launches, algorithmic bytes of all launches (each
+ * staged source read once, each target read and written once), moment-cell
+ * updates, and the persistent grid size. */
+int smp_batch_timing(const smp_batch *b, double *total_ms, double *kernel_ms, int64_t *launches,
+                     double *bytes, int64_t *mcu, int32_t *grid);
 
 #ifdef __cplusplus
 }

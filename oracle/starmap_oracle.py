@@ -204,7 +204,8 @@ def starmap_init(prob):
     par.setdefault("cn", 0.99)
     par.setdefault("bc", [0, 0, 0])
     par.setdefault("t_plot", list(np.arange(0, 11) / 10.0))
-    par.setdefault("tfinal", par["t_plot"][-1])
+    if "tfinal" not in par:
+        par["tfinal"] = par["t_plot"][-1]
     par.setdefault("mom_output", 1)
     par["closure_mod"] = "F" if "filterfunction" in par else ""
     return par

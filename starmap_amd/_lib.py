@@ -23,7 +23,10 @@ _ERRNAMES = {SMP_E_INVALID: "invalid", SMP_E_HIP: "hip", SMP_E_OOM: "oom",
 EXPORTS = ["smp_last_error", "smp_version", "smp_create", "smp_destroy", "smp_grid_types",
            "smp_component_shape", "smp_set_state", "smp_get_state", "smp_set_coeff",
            "smp_step", "smp_snapshot", "smp_interp", "smp_observables",
-           "smp_last_step_timing", "smp_stream", "smp_device_bytes", "smp_plan_stats"]
+           "smp_last_step_timing", "smp_stream", "smp_device_bytes", "smp_plan_stats",
+           "smp_set_schedule", "smp_get_schedule", "smp_checkpoint_save", "smp_checkpoint_load",
+           "smp_checkpoint_free", "smp_batch_create", "smp_batch_destroy", "smp_batch_run",
+           "smp_batch_timing"]
 
 
 class SmpError(RuntimeError):
@@ -82,6 +85,15 @@ def lib():
         "smp_stream": (vp, [vp]),
         "smp_device_bytes": (i64, [vp]),
         "smp_plan_stats": (ctypes.c_int, [vp, i32, vp, vp, vp, vp]),
+        "smp_set_schedule": (ctypes.c_int, [vp, vp, i32]),
+        "smp_get_schedule": (ctypes.c_int, [vp, vp, vp]),
+        "smp_checkpoint_save": (ctypes.c_int, [vp, i32]),
+        "smp_checkpoint_load": (ctypes.c_int, [vp, i32]),
+        "smp_checkpoint_free": (ctypes.c_int, [vp, i32]),
+        "smp_batch_create": (ctypes.c_int, [vp, i32, vp]),
+        "smp_batch_destroy": (ctypes.c_int, [vp]),
+        "smp_batch_run": (ctypes.c_int, [vp, vp, vp, i32]),
+        "smp_batch_timing": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -1,0 +1,124 @@
+"""Autotuner for the substep schedules.
+
+For a prepared ``HipSolver`` or ``BatchSolver`` it tries candidate
+schedules -- the workgroup tile shape and the LDS budget (workgroups per
+CU the schedule is made for) -- by re-making the schedules of every handle
+(``smp_set_schedule``), timing a few steps with HIP events, and keeping the
+fastest.  The state is saved to a device checkpoint before the trials and
+restored after each one, so tuning does not advance the solution.  Results
+are cached per problem signature (grid, moment count, precision of every
+handle) in a JSON file (``SMP_TUNE_CACHE``, default
+``~/.cache/starmap_amd/tune.json``).
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+
+TILES = {1: [(255, 1, 1)],
+         2: [(63, 4, 1), (127, 2, 1), (31, 8, 1)],
+         3: [(15, 4, 4), (31, 4, 2), (15, 8, 2)]}
+BLOCKS = (1, 2)
+_SLOT = 4095            # checkpoint slot used while tuning
+
+
+def _solvers(target):
+    return target.solvers if hasattr(target, "solvers") else [target]
+
+
+def signature(target):
+    items = []
+    for s in _solvers(target):
+        items.append([int(v) for v in s.n] + [int(s.n_sys), s.precision, int(len(s.gkey))])
+    return hashlib.sha1(json.dumps([type(target).__name__, items]).encode()).hexdigest()
+
+
+def _cache_path():
+    return os.environ.get("SMP_TUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "starmap_amd",
+                                       "tune.json"))
+
+
+def _load_cache():
+    try:
+        with open(_cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store_cache(key, entry):
+    path = _cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        c = _load_cache()
+        c[key] = entry
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(c, f, indent=1)
+        os.replace(tmp, path)
+    except OSError:
+        pass
+
+
+def candidates(target):
+    dim = max(int(np.sum(np.asarray(s.n) > 1)) for s in _solvers(target))
+    return [(t, b) for t in TILES[max(dim, 1)] for b in BLOCKS]
+
+
+def apply(target, tile, blocks_per_cu):
+    for s in _solvers(target):
+        s.set_schedule(tile, blocks_per_cu)
+
+
+def _time(target, steps):
+    """Kernel milliseconds of `steps` steps (HIP events)."""
+    if hasattr(target, "solvers"):
+        target.step(np.full(len(target), steps), use_graph=False)
+        return target.timing()[1]
+    target.step(target.dt_prepared, steps)
+    return target.timing()[1]
+
+
+def tune(target, steps=2, use_cache=True, verbose=False):
+    """Choose and apply the fastest schedule for a prepared target; returns
+    (tile, blocks_per_cu, {candidate: ms})."""
+    key = signature(target)
+    if use_cache:
+        hit = _load_cache().get(key)
+        if hit:
+            apply(target, tuple(hit["tile"]), hit["blocks_per_cu"])
+            return tuple(hit["tile"]), hit["blocks_per_cu"], {}
+    sol = _solvers(target)
+    for s in sol:
+        s.checkpoint_save(_SLOT)
+    times = {}
+    try:
+        for tile, b in candidates(target):
+            try:
+                apply(target, tile, b)
+            except Exception as e:        # a schedule the coupling does not fit
+                if verbose:
+                    print("autotune: skip", tile, b, e)
+                continue
+            _time(target, 1)             # first use: coefficient upload, warm-up
+            for s in sol:
+                s.checkpoint_load(_SLOT)
+            ms = _time(target, steps)
+            for s in sol:
+                s.checkpoint_load(_SLOT)
+            times[(tile, b)] = ms
+            if verbose:
+                print("autotune:", tile, b, "%.3f ms" % ms)
+    finally:
+        for s in sol:
+            s.checkpoint_load(_SLOT)
+            s.checkpoint_free(_SLOT)
+    if not times:
+        raise RuntimeError("autotune: no candidate schedule fits")
+    (tile, b), _ = min(times.items(), key=lambda kv: kv[1])
+    apply(target, tile, b)
+    _store_cache(key, {"tile": list(tile), "blocks_per_cu": b,
+                       "ms": {"%s/%d" % (t, bb): v for (t, bb), v in times.items()}})
+    return tile, b, times

@@ -102,6 +102,7 @@ struct smp_problem {
     void *red_buf = nullptr;
     void *sink = nullptr;         // 256 elements: stores of lanes without a target cell
     std::vector<hipEvent_t> events;
+    std::vector<void *> ckpt;     // device copies of the whole state
     double last_total_ms = 0, last_substep_ms = 0, last_bytes = 0;
     int64_t last_launches = 0;
     int64_t dev_bytes = 0;
@@ -236,7 +237,8 @@ int make_plan(smp_problem *h, int kind, int ncf_reserve) {
     int blocks = h->blocks_per_cu;
     int cap = 0;
     for (; blocks >= 1; --blocks) {
-        const size_t per_block = 160 * 1024 / blocks;
+        // 256 bytes of the 160 KB stay free (static LDS of the batched kernel)
+        const size_t per_block = (160 * 1024 - 256) / blocks;
         if (per_block <= fixed + 2 * slot_bytes) continue;
         cap = (int)((per_block - fixed) / slot_bytes);
         err = smp::build_plan(h->geo, h->ts, kind, cap, smp::GMAX, max_load_elems, smp::MAXL, plan);
@@ -830,6 +832,266 @@ int run_steps(smp_problem *h, double dt, int64_t n_steps) {
     return SMP_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Checkpoints of the state in device memory (slots of a full state copy)
+int checkpoint_buffer(smp_problem *h, int slot, void **out) {
+    if (slot < 0 || slot > 4096) return fail(SMP_E_INVALID, "checkpoint slot must be 0..4096");
+    if ((int)h->ckpt.size() <= slot) h->ckpt.resize(slot + 1, nullptr);
+    if (!h->ckpt[slot]) {
+        int rc = h->dalloc(&h->ckpt[slot], (size_t)h->slab * h->elem * h->geo.n_sys);
+        if (rc) return rc;
+    }
+    *out = h->ckpt[slot];
+    return SMP_OK;
+}
+
+// algorithmic bytes of one substep launch of kind `kind` (0: odd targets):
+// every source component read once, every target read and written once
+double substep_bytes(const smp_problem *h, int kind) {
+    const double cells = (double)h->geo.n[0] * h->geo.n[1] * h->geo.n[2];
+    const double ne = (double)h->geo.even.size(), no = (double)h->geo.odd.size();
+    return (double)h->elem * cells * (kind == 1 ? no + 2 * ne : ne + 2 * no);
+}
+
+}  // namespace
+
+struct smp_batch {
+    std::vector<smp_problem *> hs;
+    int device = 0;
+    int precision = SMP_FP32;
+    hipStream_t stream = nullptr;
+    // device tables: entries of every launch, one descriptor and one counter
+    // per launch
+    void *entries = nullptr;
+    size_t entries_cap = 0;
+    void *launch = nullptr;
+    int *counters = nullptr;
+    int launch_cap = 0;
+    // captured launch sequence and the structure it was captured for
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int g_launches = -1, g_grid = 0, g_nt = 0;
+    size_t g_lds = 0;
+    bool g_gen = false;
+    std::vector<hipEvent_t> events;
+    double last_ms = 0, last_kernel_ms = 0, last_bytes = 0;
+    int64_t last_launches = 0, last_mcu = 0;
+    int grid = 0;
+    size_t lds = 0;
+
+    ~smp_batch() {
+        (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (void *p : {entries, launch, (void *)counters})
+            if (p) (void)hipFree(p);
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+template <typename T>
+int batch_run(smp_batch *b, const double *dt, const int64_t *ns, int use_graph) {
+    const int n = (int)b->hs.size();
+    int64_t max_n = 0;
+    for (int i = 0; i < n; ++i) {
+        if (!(dt[i] > 0) || !std::isfinite(dt[i]))
+            return fail(SMP_E_INVALID, "dt must be positive and finite");
+        if (ns[i] < 0) return fail(SMP_E_INVALID, "n_steps must be >= 0");
+        max_n = std::max(max_n, ns[i]);
+    }
+    // coefficients of every problem for its dt (the problems' own launches
+    // are finished first)
+    for (int i = 0; i < n; ++i) {
+        smp_problem *h = b->hs[i];
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        if (ns[i] == 0) continue;
+        if (h->decay_dirty || !(h->dt_decay == dt[i])) {
+            int rc = refresh_decay<T>(h, dt[i]);
+            if (rc) return rc;
+        }
+        if (h->coef_dirty) {
+            int rc = refresh_coefs<T>(h);
+            if (rc) return rc;
+        }
+    }
+    b->last_ms = b->last_kernel_ms = b->last_bytes = 0;
+    b->last_launches = b->last_mcu = 0;
+    if (max_n == 0) return SMP_OK;
+    // launch L: L = 0 the first odd substep of every problem; L = 2k+1 the
+    // even substep of step k; L = 2k (k >= 1) the odd substep that ends step
+    // k-1 and, fused, begins step k, or ends the last step
+    struct Var {
+        smp::SubstepArgs<T> odd_first, even, odd_fused, odd_last;
+    };
+    std::vector<Var> var(n);
+    int nt = 0;
+    bool gen = false;
+    size_t lds = 0;
+    for (int i = 0; i < n; ++i) {
+        smp_problem *h = b->hs[i];
+        if (ns[i] == 0) continue;
+        var[i].odd_first = make_args<T>(h, 0, dt[i], 1, 0, 0);
+        var[i].even = make_args<T>(h, 1, dt[i], 2, 0, 1);
+        var[i].odd_fused = make_args<T>(h, 0, dt[i], 2, 1, 0);
+        var[i].odd_last = make_args<T>(h, 0, dt[i], 1, 1, 1);
+        for (int kind = 0; kind < 2; ++kind) {
+            nt = std::max(nt, h->plan[kind].max_terms);
+            gen = gen || h->gen[kind];
+            lds = std::max(lds, h->lds_bytes[kind]);
+        }
+    }
+    const int n_launch = (int)(2 * max_n + 1);
+    std::vector<smp::BatchEntry<T>> ent;
+    std::vector<smp::BatchLaunch<T>> desc(n_launch);
+    std::vector<size_t> first(n_launch + 1, 0);
+    double bytes = 0;
+    for (int L = 0; L < n_launch; ++L) {
+        first[L] = ent.size();
+        const int64_t k = L / 2;
+        for (int i = 0; i < n; ++i) {
+            if (ns[i] == 0) continue;
+            const smp::SubstepArgs<T> *a = nullptr;
+            if (L == 0)
+                a = &var[i].odd_first;
+            else if (L & 1)
+                a = k < ns[i] ? &var[i].even : nullptr;
+            else
+                a = k < ns[i] ? &var[i].odd_fused : (k == ns[i] ? &var[i].odd_last : nullptr);
+            if (!a) continue;
+            smp::BatchEntry<T> e;
+            std::memset(&e, 0, sizeof(e));
+            e.a = *a;
+            e.pad = i;
+            ent.push_back(e);
+            bytes += substep_bytes(b->hs[i], (L & 1) ? 1 : 0);
+        }
+        // heaviest tiles first (most groups per tile), so that the last
+        // units handed out are the cheap ones
+        std::stable_sort(ent.begin() + first[L], ent.end(),
+                         [](const smp::BatchEntry<T> &x, const smp::BatchEntry<T> &y) {
+                             return x.a.ngroups > y.a.ngroups;
+                         });
+        int32_t u = 0;
+        for (size_t q = first[L]; q < ent.size(); ++q) {
+            ent[q].unit0 = u;
+            u += ent[q].a.n_tiles;
+        }
+        desc[L].n_entries = (int32_t)(ent.size() - first[L]);
+        desc[L].n_units = u;
+    }
+    first[n_launch] = ent.size();
+    // device tables (kept between calls: a captured graph refers to them)
+    HIP_TRY(hipSetDevice(b->device));
+    const size_t ebytes = sizeof(smp::BatchEntry<T>) * std::max<size_t>(1, ent.size());
+    if (ebytes > b->entries_cap || n_launch > b->launch_cap) {
+        if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        b->gexec = nullptr;
+        b->graph = nullptr;
+        b->g_launches = -1;
+        for (void *p : {b->entries, b->launch, (void *)b->counters})
+            if (p) (void)hipFree(p);
+        b->entries = b->launch = nullptr;
+        b->counters = nullptr;
+        b->entries_cap = ebytes + ebytes / 4;
+        b->launch_cap = n_launch + n_launch / 4 + 1;
+        HIP_TRY(hipMalloc(&b->entries, b->entries_cap));
+        HIP_TRY(hipMalloc(&b->launch, sizeof(smp::BatchLaunch<T>) * b->launch_cap));
+        HIP_TRY(hipMalloc((void **)&b->counters, sizeof(int) * b->launch_cap));
+    }
+    smp::BatchEntry<T> *dent = (smp::BatchEntry<T> *)b->entries;
+    for (int L = 0; L < n_launch; ++L) {
+        desc[L].entries = dent + first[L];
+        desc[L].counter = b->counters + L;
+    }
+    HIP_TRY(hipMemcpyAsync(b->entries, ent.data(), sizeof(smp::BatchEntry<T>) * ent.size(),
+                           hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemcpyAsync(b->launch, desc.data(), sizeof(smp::BatchLaunch<T>) * n_launch,
+                           hipMemcpyHostToDevice, b->stream));
+    // persistent grid: every CU holds as many workgroups as fit
+    int ncu = 0, per_cu = 0;
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device));
+    HIP_TRY(smp::batch_occupancy<T>(nt, gen, lds, &per_cu));
+    if (per_cu < 1) return fail(SMP_E_INVALID, "the batched kernel does not fit on a CU");
+    const int grid = ncu * per_cu;
+    b->grid = grid;
+    b->lds = lds;
+    const smp::BatchLaunch<T> *dl = (const smp::BatchLaunch<T> *)b->launch;
+    auto issue = [&](bool timed) -> int {
+        HIP_TRY(smp::launch_fill(b->counters, n_launch, 0, b->stream));
+        for (int L = 0; L < n_launch; ++L) {
+            if (timed) HIP_TRY(hipEventRecord(b->events[2 + 2 * L], b->stream));
+            HIP_TRY(smp::launch_batch<T>(dl + L, nt, gen, grid, lds, b->stream));
+            if (timed) HIP_TRY(hipEventRecord(b->events[3 + 2 * L], b->stream));
+        }
+        return SMP_OK;
+    };
+    const bool timed = !use_graph && n_launch <= 4096;
+    while ((int)b->events.size() < (timed ? 2 * n_launch + 2 : 2)) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        b->events.push_back(e);
+    }
+    if (use_graph) {
+        const bool same = b->gexec && b->g_launches == n_launch && b->g_grid == grid &&
+                          b->g_lds == lds && b->g_nt == nt && b->g_gen == gen;
+        if (!same) {
+            if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+            if (b->graph) (void)hipGraphDestroy(b->graph);
+            b->gexec = nullptr;
+            b->graph = nullptr;
+            HIP_TRY(hipStreamSynchronize(b->stream));
+            HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            int rc = issue(false);
+            hipGraph_t g = nullptr;
+            hipError_t ec = hipStreamEndCapture(b->stream, &g);
+            if (rc) return rc;
+            HIP_TRY(ec);
+            b->graph = g;
+            HIP_TRY(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+            b->g_launches = n_launch;
+            b->g_grid = grid;
+            b->g_lds = lds;
+            b->g_nt = nt;
+            b->g_gen = gen;
+        }
+        HIP_TRY(hipEventRecord(b->events[0], b->stream));
+        HIP_TRY(hipGraphLaunch(b->gexec, b->stream));
+        HIP_TRY(hipEventRecord(b->events[1], b->stream));
+    } else {
+        HIP_TRY(hipEventRecord(b->events[0], b->stream));
+        int rc = issue(timed);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(b->events[1], b->stream));
+    }
+    HIP_TRY(hipEventSynchronize(b->events[1]));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, b->events[0], b->events[1]));
+    b->last_ms = ms;
+    double kms = 0;
+    if (timed) {
+        for (int L = 0; L < n_launch; ++L) {
+            float m = 0;
+            HIP_TRY(hipEventElapsedTime(&m, b->events[2 + 2 * L], b->events[3 + 2 * L]));
+            kms += m;
+        }
+    } else {
+        kms = ms;
+    }
+    b->last_kernel_ms = kms;
+    b->last_launches = n_launch;
+    b->last_bytes = bytes;
+    for (int i = 0; i < n; ++i) {
+        const smp_problem *h = b->hs[i];
+        b->last_mcu += (int64_t)h->geo.n_sys * h->geo.n[0] * h->geo.n[1] * h->geo.n[2] * ns[i];
+    }
+    return SMP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -977,4 +1239,143 @@ int smp_plan_stats(const smp_problem *h, int32_t kind, int32_t *n_groups, int32_
     return SMP_OK;
 }
 
+int smp_set_schedule(smp_problem *h, const int32_t *tile, int32_t blocks_per_cu) {
+    if (!h) return fail(SMP_E_INVALID, "null handle");
+    if (blocks_per_cu < 0 || blocks_per_cu > 8) return fail(SMP_E_INVALID, "blocks_per_cu must be 0..8");
+    if (!h->host_only) {
+        HIP_TRY(hipSetDevice(h->device));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
+    smp::TileShape ts;
+    if (tile && (tile[0] || tile[1] || tile[2])) {
+        std::string err = smp::make_tile(h->geo, std::max(1, tile[0]), std::max(1, tile[1]),
+                                         std::max(1, tile[2]), ts);
+        if (!err.empty()) return fail(SMP_E_INVALID, err);
+    } else {
+        ts = smp::choose_tile(h->geo, h->elem);
+    }
+    if ((ts.slot_elems * (h->elem / 4) + 63) / 64 > smp::CPS_MAX)
+        return fail(SMP_E_INVALID, "a tile with its halo exceeds 4 KB per component");
+    const smp::TileShape old_ts = h->ts;
+    const int old_blocks = h->blocks_per_cu;
+    h->ts = ts;
+    if (blocks_per_cu) h->blocks_per_cu = blocks_per_cu;
+    for (int kind = 0; kind < 2; ++kind) {
+        int rc = make_plan(h, kind, h->ncf_cap[kind]);
+        if (rc) {
+            // keep the handle usable with its previous schedule
+            const std::string msg = g_err;
+            h->ts = old_ts;
+            h->blocks_per_cu = old_blocks;
+            for (int k2 = 0; k2 < 2; ++k2) make_plan(h, k2, h->ncf_cap[k2]);
+            h->coef_dirty = true;
+            return fail(rc, msg);
+        }
+    }
+    h->n_tiles = 1;
+    for (int e = 0; e < 3; ++e) {
+        h->ntile[e] = (h->E[e] + h->ts.tile[e] - 1) / h->ts.tile[e];
+        h->n_tiles *= h->ntile[e];
+    }
+    h->coef_dirty = true;
+    return SMP_OK;
+}
+
+int smp_get_schedule(const smp_problem *h, int32_t *tile, int32_t *blocks_per_cu) {
+    if (!h) return fail(SMP_E_INVALID, "null handle");
+    if (tile)
+        for (int e = 0; e < 3; ++e) tile[e] = h->ts.tile[e];
+    if (blocks_per_cu) *blocks_per_cu = h->blocks_per_cu;
+    return SMP_OK;
+}
+
+int smp_checkpoint_save(smp_problem *h, int32_t slot) {
+    if (!h) return fail(SMP_E_INVALID, "null handle");
+    REQUIRE_DEVICE(h);
+    HIP_TRY(hipSetDevice(h->device));
+    void *dst = nullptr;
+    int rc = checkpoint_buffer(h, slot, &dst);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(dst, h->U, (size_t)h->slab * h->elem * h->geo.n_sys,
+                           hipMemcpyDeviceToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return SMP_OK;
+}
+
+int smp_checkpoint_load(smp_problem *h, int32_t slot) {
+    if (!h) return fail(SMP_E_INVALID, "null handle");
+    REQUIRE_DEVICE(h);
+    if (slot < 0 || slot >= (int)h->ckpt.size() || !h->ckpt[slot])
+        return fail(SMP_E_STATE, "checkpoint slot " + std::to_string(slot) + " was never saved");
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipMemcpyAsync(h->U, h->ckpt[slot], (size_t)h->slab * h->elem * h->geo.n_sys,
+                           hipMemcpyDeviceToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return SMP_OK;
+}
+
+int smp_checkpoint_free(smp_problem *h, int32_t slot) {
+    if (!h) return fail(SMP_E_INVALID, "null handle");
+    if (!h->host_only) {
+        HIP_TRY(hipSetDevice(h->device));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
+    for (int s = 0; s < (int)h->ckpt.size(); ++s) {
+        if (slot >= 0 && s != slot) continue;
+        h->dfree(h->ckpt[s]);
+        h->ckpt[s] = nullptr;
+    }
+    return SMP_OK;
+}
+
+int smp_batch_create(smp_problem *const *handles, int32_t n, smp_batch **out) {
+    g_err.clear();
+    if (!handles || !out || n < 1) return fail(SMP_E_INVALID, "a batch needs at least one handle");
+    *out = nullptr;
+    std::unique_ptr<smp_batch> b(new smp_batch());
+    for (int i = 0; i < n; ++i) {
+        smp_problem *h = handles[i];
+        if (!h) return fail(SMP_E_INVALID, "null handle in the batch");
+        if (h->host_only) return fail(SMP_E_STATE, "handle was created without a device (device = -1)");
+        if (i > 0 && (h->device != handles[0]->device || h->precision != handles[0]->precision))
+            return fail(SMP_E_INVALID, "batched handles must share device and precision");
+        if (h->f_position == SMP_FILTER_FULL)
+            return fail(SMP_E_INVALID, "batched stepping supports no filter or the substep filter");
+        for (int k = 0; k < i; ++k)
+            if (handles[k] == h) return fail(SMP_E_INVALID, "a handle appears twice in the batch");
+        b->hs.push_back(h);
+    }
+    b->device = handles[0]->device;
+    b->precision = handles[0]->precision;
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    *out = b.release();
+    return SMP_OK;
+}
+
+int smp_batch_destroy(smp_batch *b) {
+    delete b;
+    return SMP_OK;
+}
+
+int smp_batch_run(smp_batch *b, const double *dt, const int64_t *n_steps, int32_t use_graph) {
+    if (!b || !dt || !n_steps) return fail(SMP_E_INVALID, "null argument");
+    HIP_TRY(hipSetDevice(b->device));
+    return b->precision == SMP_FP64 ? batch_run<double>(b, dt, n_steps, use_graph)
+                                    : batch_run<float>(b, dt, n_steps, use_graph);
+}
+
+int smp_batch_timing(const smp_batch *b, double *total_ms, double *kernel_ms, int64_t *launches,
+                     double *bytes, int64_t *mcu, int32_t *grid) {
+    if (!b) return fail(SMP_E_INVALID, "null batch");
+    if (total_ms) *total_ms = b->last_ms;
+    if (kernel_ms) *kernel_ms = b->last_kernel_ms;
+    if (launches) *launches = b->last_launches;
+    if (bytes) *bytes = b->last_bytes;
+    if (mcu) *mcu = b->last_mcu;
+    if (grid) *grid = b->grid;
+    return SMP_OK;
+}
+
 }  // extern "C"
+

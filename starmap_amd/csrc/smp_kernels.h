@@ -41,6 +41,24 @@ struct SubstepArgs {
     int32_t fslot[2];         // filter slot of application 0 and 1
 };
 
+// One problem's substep in a batched launch: its arguments and the index of
+// its first work unit (units unit0 .. unit0 + a.n_tiles - 1 are its tiles).
+template <typename T>
+struct BatchEntry {
+    SubstepArgs<T> a;
+    int32_t unit0;
+    int32_t pad;
+};
+
+// Device-resident descriptor of one batched launch.
+template <typename T>
+struct BatchLaunch {
+    const BatchEntry<T> *entries;
+    int32_t n_entries;
+    int32_t n_units;
+    int *counter;             // work-unit counter, zero before the launch
+};
+
 // LDS bytes the kernel needs besides the source slots and the staged
 // coefficient fields (ncf * threads elements)
 size_t substep_lds_overhead();
@@ -51,6 +69,19 @@ size_t substep_lds_overhead();
 template <typename T>
 hipError_t launch_substep(const SubstepArgs<T> &a, int nt, bool gen, int threads,
                           size_t lds_bytes, hipStream_t s);
+
+// Persistent batched substep launch over `grid` workgroups; bl is a device
+// pointer.  nt / gen as for launch_substep, over all entries.
+template <typename T>
+hipError_t launch_batch(const BatchLaunch<T> *bl, int nt, bool gen, int grid, size_t lds_bytes,
+                        hipStream_t s);
+
+// workgroups of the batched kernel resident per CU with lds_bytes of LDS
+template <typename T>
+hipError_t batch_occupancy(int nt, bool gen, size_t lds_bytes, int *blocks);
+
+// p[0..n) = v (a kernel, so that it can be captured in a graph)
+hipError_t launch_fill(int *p, int n, int v, hipStream_t s);
 
 template <typename T>
 hipError_t launch_decay(const T *sig, T *E, int64_t count, double dt, hipStream_t s);

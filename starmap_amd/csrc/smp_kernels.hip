@@ -57,14 +57,15 @@ typedef __attribute__((address_space(3))) void *lds_ptr_t;
 // barrier per group.  Only the copy waves have copies in flight, so the cell
 // waves never wait on them before reading LDS.  The schedule never lets group
 // g+1 copy into a slot that group g reads.
+//
+// NAPP = 0 takes the number of half-step applications from a.napp (the
+// batched kernel, whose work units mix substeps of different kinds).
 template <typename T, int NT, int NAPP, bool GEN>
-__global__ __launch_bounds__(KTHREADS) void substep_kernel(const SubstepArgs<T> a) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
+__device__ __forceinline__ void substep_tile(const SubstepArgs<T> &a, const int tile,
+                                             unsigned char *smem_raw) {
     unsigned char *ring = smem_raw;
     T *cfa = reinterpret_cast<T *>(smem_raw + 3 * BLOB_SZ);
     T *lds = cfa + a.ncf * 256;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    if (tile >= a.n_tiles) return;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const bool copier = tid >= 256;
@@ -249,7 +250,8 @@ __global__ __launch_bounds__(KTHREADS) void substep_kernel(const SubstepArgs<T> 
             const A wq = w + coef(h1.x, r.Q_p, 0.0);
             A u = (A)u_in[i];
 #pragma unroll
-            for (int app = 0; app < NAPP; ++app) {
+            for (int app = 0; app < (NAPP ? NAPP : 2); ++app) {
+                if (NAPP == 0 && app >= a.napp) break;
                 // U + dt/2*(W+Q - s.*U).*E   (starmap_solver.m:352-354)
                 u = u + (a.half_dt * (wq - sig * u)) * E;
                 const bool s1 = a.fslot[app] != 0;
@@ -310,6 +312,49 @@ __global__ __launch_bounds__(KTHREADS) void substep_kernel(const SubstepArgs<T> 
 }
 
 template <typename T, int NT, int NAPP, bool GEN>
+__global__ __launch_bounds__(KTHREADS) void substep_kernel(const SubstepArgs<T> a) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tile >= a.n_tiles) return;
+    substep_tile<T, NT, NAPP, GEN>(a, tile, smem_raw);
+}
+
+// Batched persistent kernel: one launch runs a substep of many independent
+// problems.  Work units are (entry, tile) pairs, entries ordered by cost per
+// tile, heaviest first; every workgroup takes the next unit from a counter
+// until none is left, so uneven problems balance over the CUs.
+template <typename T, int NT, bool GEN>
+__global__ __launch_bounds__(KTHREADS) void batch_kernel(const BatchLaunch<T> *bl) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ int s_unit;
+    const int n_units = bl->n_units;
+    const int n_entries = bl->n_entries;
+    const BatchEntry<T> *ent = bl->entries;
+    for (;;) {
+        if (threadIdx.x == 0) s_unit = atomicAdd(bl->counter, 1);
+        __syncthreads();
+        const int u = __builtin_amdgcn_readfirstlane(s_unit);
+        __syncthreads();
+        if (u >= n_units) break;
+        // entry holding unit u: the last entry whose first unit is <= u
+        int lo = 0, hi = n_entries - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (ent[mid].unit0 <= u)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        substep_tile<T, NT, 0, GEN>(ent[lo].a, u - ent[lo].unit0, smem_raw);
+    }
+}
+
+__global__ void fill_kernel(int *p, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+template <typename T, int NT, int NAPP, bool GEN>
 static hipError_t launch_v(const SubstepArgs<T> &a, int threads, size_t lds_bytes, hipStream_t s) {
     static bool attr_set = false;
     if (!attr_set) {
@@ -343,6 +388,42 @@ hipError_t launch_substep(const SubstepArgs<T> &a, int nt, bool gen, int threads
         return hipErrorInvalidValue;
     if (nt <= 8) return launch_nt<T, 8>(a, gen, threads, lds_bytes, s);
     return launch_nt<T, MAXT>(a, gen, threads, lds_bytes, s);
+}
+
+// (the dynamic LDS limit of batch_kernel is raised by batch_occupancy, which
+// runs before any launch and outside graph capture)
+template <typename T, int NT, bool GEN>
+static hipError_t launch_b(const BatchLaunch<T> *bl, int grid, size_t lds_bytes, hipStream_t s) {
+    hipLaunchKernelGGL((batch_kernel<T, NT, GEN>), dim3(grid), dim3(KTHREADS), lds_bytes, s, bl);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_batch(const BatchLaunch<T> *bl, int nt, bool gen, int grid, size_t lds_bytes,
+                        hipStream_t s) {
+    if (grid < 1 || lds_bytes > 160 * 1024 - 64) return hipErrorInvalidValue;
+    if (nt <= 8) return gen ? launch_b<T, 8, true>(bl, grid, lds_bytes, s)
+                            : launch_b<T, 8, false>(bl, grid, lds_bytes, s);
+    return gen ? launch_b<T, MAXT, true>(bl, grid, lds_bytes, s)
+               : launch_b<T, MAXT, false>(bl, grid, lds_bytes, s);
+}
+
+template <typename T>
+hipError_t batch_occupancy(int nt, bool gen, size_t lds_bytes, int *blocks) {
+    const void *f;
+    if (nt <= 8)
+        f = gen ? (const void *)batch_kernel<T, 8, true> : (const void *)batch_kernel<T, 8, false>;
+    else
+        f = gen ? (const void *)batch_kernel<T, MAXT, true> : (const void *)batch_kernel<T, MAXT, false>;
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, f, KTHREADS, lds_bytes);
+}
+
+hipError_t launch_fill(int *p, int n, int v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -498,7 +579,10 @@ hipError_t launch_reduce_final(const double *part, int blocks, double *out3, hip
     template hipError_t launch_interp<T>(const T *, const T *, double *, int64_t, double,     \
                                          hipStream_t);                                         \
     template hipError_t launch_reduce<T>(const T *, const ReduceGeom &, double *, int,         \
-                                         hipStream_t);
+                                         hipStream_t);                                         \
+    template hipError_t launch_batch<T>(const BatchLaunch<T> *, int, bool, int, size_t,        \
+                                        hipStream_t);                                          \
+    template hipError_t batch_occupancy<T>(int, bool, size_t, int *);
 
 SMP_INSTANTIATE(double)
 SMP_INSTANTIATE(float)

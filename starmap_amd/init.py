@@ -133,6 +133,7 @@ def starmap_init(prob):
         # that takes the component index (starmap_init.m:218-221)
         cnt = int(na >= 0) + int(na > 4) * (n_sys - 1)
         par["source_ind"] = list(range(1, cnt + 1))
-    par.setdefault("tfinal", par["t_plot"][-1])
+    if "tfinal" not in par:
+        par["tfinal"] = par["t_plot"][-1]
     par["closure_mod"] = "F" if "filterfunction" in par else ""
     return par

@@ -188,3 +188,76 @@ def boxes_2d(n=50, n_mom=9, closure="P", tfinal=0.2):
         "tfinal": tfinal,
         "output": None,
     }
+
+
+BATCH_SIDES = tuple(range(64, 513, 32))     # 64, 96, ..., 512
+BATCH_ORDERS = tuple(range(3, 40, 2))       # odd N = 3, 5, ..., 39
+
+
+def batch_draws(n_instances=64, seed=2025, sides=BATCH_SIDES, orders=BATCH_ORDERS):
+    """Seeded parameters of the batched sweep: per instance a grid side, an
+    odd moment order N, a filter strength f_eff ~ U[0,20] and, on a 4x4 block
+    partition of the domain, sigma_a and sigma_s0 log-uniform on [0.01,10].
+    One numpy default_rng(seed) draws, in this order: sides (choice), orders
+    (choice), f_eff (uniform), then per instance 16 sigma_a and 16 sigma_s0
+    exponents (uniform on [-2,1])."""
+    rng = np.random.default_rng(seed)
+    side = rng.choice(np.asarray(sides), size=n_instances)
+    order = rng.choice(np.asarray(orders), size=n_instances)
+    f_eff = rng.uniform(0.0, 20.0, size=n_instances)
+    blocks = []
+    for _ in range(n_instances):
+        sa = 10.0 ** rng.uniform(-2.0, 1.0, size=(4, 4))
+        ss = 10.0 ** rng.uniform(-2.0, 1.0, size=(4, 4))
+        blocks.append((sa, ss))
+    return [dict(side=int(side[i]), n_mom=int(order[i]), f_eff=float(f_eff[i]),
+                 sigma_a=blocks[i][0], sigma_s0=blocks[i][1]) for i in range(n_instances)]
+
+
+def _block4(table, x, y):
+    # 4x4 blocks of width 0.5 on [-1,1]^2 with the ceil() convention of
+    # starmap_2d_ex_lattice.m:50-53: a face on a block boundary belongs to
+    # the block on its left/lower side; the boundary at -1 to block 1
+    bx = np.clip(np.ceil((np.asarray(x) + 1.0) / 0.5), 1, 4).astype(np.int64)
+    by = np.clip(np.ceil((np.asarray(y) + 1.0) / 0.5), 1, 4).astype(np.int64)
+    return table[bx - 1, by - 1]
+
+
+def batch_instance(side, n_mom, f_eff, sigma_a, sigma_s0, tfinal=1.0, std=0.05,
+                   filter_p=4, closure="P"):
+    """One instance of the batched sweep: 2D P_N on the periodic [-1,1]^2
+    with side x side cells, unit-mass Gaussian zeroth moment (std 0.05),
+    piecewise-constant sigma_a / sigma_s0 on 4x4 blocks, exponential filter
+    of order filter_p and strength f_eff at 'substep' position,
+    dt = 0.5*dx (cn = lambda_max)."""
+    lam = legendre_max_root(n_mom + 1)
+    sa, ss = np.asarray(sigma_a), np.asarray(sigma_s0)
+    return {
+        "name": "Batched instance",
+        "closure": closure,
+        "n_mom": n_mom,
+        "ic": gaussian_2d(std),
+        "sigma_a": (lambda x, y: _block4(sa, x, y)),
+        "sigma_s0": (lambda x, y: _block4(ss, x, y)),
+        "ax": [-1.0, 1.0, -1.0, 1.0],
+        "n": [side, side],
+        "bc": [0, 0],
+        "cn": lam,
+        "t_plot": [],
+        "tfinal": tfinal,
+        "mom_output": 1,
+        "output": None,
+        "filterfunction": exp_filter(filter_p, f_eff),
+        "f_position": "substep",
+    }
+
+
+def batch_sweep(n_instances=64, seed=2025, tfinal=1.0, side_div=1, max_order=None):
+    """The batched sweep: 64 seeded instances (batch_draws).  side_div
+    divides every side (reduced sizes for tests); max_order caps N."""
+    out = []
+    for d in batch_draws(n_instances, seed):
+        n_mom = d["n_mom"] if max_order is None else min(d["n_mom"], max_order)
+        out.append(batch_instance(max(4, d["side"] // side_div), n_mom, d["f_eff"],
+                                  d["sigma_a"], d["sigma_s0"], tfinal=tfinal))
+    return out

@@ -131,6 +131,7 @@ class HipSolver:
         h = ctypes.c_void_p()
         _lib.check(L.smp_create(ctypes.byref(d), ctypes.byref(h)))
         self.handle = h
+        self._used = False          # stepped, or given an initial state
         gt = np.zeros(3 * n_sys, dtype=np.int32)
         _lib.check(L.smp_grid_types(h, gt.ctypes.data))
         self.gt = gt.reshape(n_sys, 3)
@@ -192,6 +193,7 @@ class HipSolver:
                                                 field.ctypes.data, 0.0, field.size))
 
     def step(self, dt, n_steps):
+        self._used = True
         _lib.check(_lib.lib().smp_step(self.handle, float(dt), int(n_steps)))
 
     def timing(self):
@@ -212,6 +214,29 @@ class HipSolver:
                                               ctypes.byref(bad)))
         return s, q, bool(bad.value)
 
+    def set_schedule(self, tile=None, blocks_per_cu=0):
+        """Re-make the substep schedules for another tile shape / LDS
+        budget; state and coefficients are kept."""
+        t = np.zeros(3, dtype=np.int32)
+        if tile is not None:
+            t[:len(tile)] = tile
+        _lib.check(_lib.lib().smp_set_schedule(self.handle, t.ctypes.data, int(blocks_per_cu)))
+
+    def get_schedule(self):
+        t = np.zeros(3, dtype=np.int32)
+        b = ctypes.c_int32()
+        _lib.check(_lib.lib().smp_get_schedule(self.handle, t.ctypes.data, ctypes.byref(b)))
+        return tuple(int(v) for v in t), b.value
+
+    def checkpoint_save(self, slot):
+        _lib.check(_lib.lib().smp_checkpoint_save(self.handle, int(slot)))
+
+    def checkpoint_load(self, slot):
+        _lib.check(_lib.lib().smp_checkpoint_load(self.handle, int(slot)))
+
+    def checkpoint_free(self, slot=-1):
+        _lib.check(_lib.lib().smp_checkpoint_free(self.handle, int(slot)))
+
     def plan_stats(self, kind):
         v = [ctypes.c_int32() for _ in range(4)]
         _lib.check(_lib.lib().smp_plan_stats(self.handle, kind, *[ctypes.byref(x) for x in v]))
@@ -220,17 +245,23 @@ class HipSolver:
     # -- setup of the reference's fields ---------------------------------
     def initial_state(self):
         """U{j} = ic(...) on the grid of j; only j==1 unless ic takes k
-        (starmap_solver.m:140-146).  All-zero components are left at the
-        zero state the handle starts with."""
+        (starmap_solver.m:140-146).  All-zero components keep the zero state
+        a new handle starts with; a handle that was stepped or written
+        before has them zeroed."""
         par = self.par
         nic = nargin(par["ic"])
         for j in range(self.n_sys):
-            if not (nic > 3 or j == 0):
-                continue
-            X, Y, Z = self.grid(self.gkey[j])
-            v = X * 0 + capargs(par["ic"], X, Y, Z, j + 1)
-            if np.any(v != 0):
+            v = None
+            if nic > 3 or j == 0:
+                X, Y, Z = self.grid(self.gkey[j])
+                v = X * 0 + capargs(par["ic"], X, Y, Z, j + 1)
+                if not np.any(v != 0):
+                    v = None
+            if v is not None:
                 self.set_state(j, v)
+            elif self._used:
+                self.set_state(j, np.zeros(self.shape(j)))
+        self._used = True
 
     def evaluate_materials(self, t_eval, evaluate):
         """Evaluate sA (grid 111 and the grids of sg) and sS, sT on the
@@ -313,6 +344,7 @@ class HipSolver:
         self.upload_source(dt - dt / 2)
         if self.flag_filter:
             self.upload_filter(dt, dt)
+        self.dt_prepared = dt
         return dt
 
     # -- the time loop ----------------------------------------------------

@@ -119,3 +119,58 @@ def test_invalid_coupling_is_rejected():
     bad["My"] = My
     with pytest.raises(SmpError, match="not bipartite"):
         HipSolver(bad, device=-1)
+
+
+def test_batch_and_schedule_host_checks():
+    """Batched stepping rejects host-only handles and bad groups before any
+    device call; re-making a schedule works on a host-only handle, and a
+    tile the coupling or LDS cannot take leaves the previous one in force."""
+    from starmap_amd.solver import HipSolver
+    from starmap_amd._lib import SmpError
+    L = _lib.lib()
+    par = I.starmap_init({"closure": "P", "n_mom": 5, "ax": [0, 1, 0, 1], "n": [20, 20],
+                          "bc": [0, 0]})
+    s = HipSolver(par, device=-1)
+    try:
+        arr = (ctypes.c_void_p * 1)(s.handle.value)
+        b = ctypes.c_void_p()
+        assert L.smp_batch_create(arr, 1, ctypes.byref(b)) == _lib.SMP_E_STATE
+        assert L.smp_batch_create(arr, 0, ctypes.byref(b)) == _lib.SMP_E_INVALID
+        assert L.smp_batch_create(None, 1, ctypes.byref(b)) == _lib.SMP_E_INVALID
+        assert s.get_schedule() == ((63, 4, 1), 1)
+        before = [s.plan_stats(k) for k in (0, 1)]
+        s.set_schedule((127, 2), 2)
+        assert s.get_schedule() == ((127, 2, 1), 2)
+        with pytest.raises(SmpError, match="power of two"):
+            s.set_schedule((30, 4), 1)
+        with pytest.raises(SmpError, match="at most 256 cells"):
+            s.set_schedule((63, 8), 1)
+        assert s.get_schedule() == ((127, 2, 1), 2)
+        s.set_schedule(None, 1)
+        assert [s.plan_stats(k) for k in (0, 1)] == before
+        with pytest.raises(SmpError, match="without a device"):
+            s.checkpoint_save(0)
+    finally:
+        s.close()
+
+
+def test_batch_step_counts_follow_the_reference_loop():
+    from starmap_amd.batch import count_steps
+    # t = 0; while t < tfinal: t = t + dt
+    for dt, tf in [(1 / 96, 1.0), (1 / 64, 1.0), (0.1, 0.3), (0.0035, 3.2)]:
+        t, n = 0.0, 0
+        while t < tf:
+            t += dt
+            n += 1
+        assert count_steps(dt, tf) == n
+    assert count_steps(1 / 64, 1.0) == 64
+
+
+def test_batch_sweep_draws_are_seeded():
+    from starmap_amd import problems as P
+    a, b = P.batch_draws(64), P.batch_draws(64)
+    assert [x["side"] for x in a] == [x["side"] for x in b]
+    assert {x["side"] for x in a} <= set(P.BATCH_SIDES)
+    assert {x["n_mom"] for x in a} <= set(P.BATCH_ORDERS)
+    assert all(0 <= x["f_eff"] <= 20 for x in a)
+    assert all(0.01 <= x["sigma_a"].min() and x["sigma_a"].max() <= 10 for x in a)
