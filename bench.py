@@ -15,7 +15,9 @@ library's stream), the FP32-state variant with its error against the FP64
 state after the same steps, and a CPU baseline (the FP64 oracle on the host
 cores, a bounded sample of the same workload).  A second leg ("gauss3d")
 measures the 3D P_7 (64 moments) 200^3 periodic Gaussian problem in FP64 and
-FP32 state.
+FP32 state; a third ("batched") 64 seeded 2D problems of uneven size, order
+and filter strength in FP32 state in the persistent batched kernel, against
+the FP64 plain rate and each problem's FP64 solve.
 """
 import argparse
 import json
@@ -131,6 +133,66 @@ def gauss3d_leg(n, n_mom, steps, warmup):
     return leg
 
 
+def batched_leg(fp64_plain_value, n_instances=64, check=True):
+    """The batched sweep: n_instances seeded 2D P_N problems (sides 64..512,
+    odd N 3..39, filter strength U[0,20], sigma_a/sigma_s0 log-uniform on
+    4x4 blocks, Gaussian initial zeroth moment, periodic [-1,1]^2,
+    t_final 1.0, dt = 0.5*dx) in FP32 state, stepped to their final times by
+    the persistent batched kernel with a captured hipGraph, on the schedule
+    the autotuner picks; each problem checked against its own FP64 solve."""
+    from starmap_amd import autotune, init
+    from starmap_amd import problems as P
+    from starmap_amd.batch import BatchSolver
+    from starmap_amd.solver import HipSolver
+    pars = [init.starmap_init(p) for p in P.batch_sweep(n_instances=n_instances)]
+    log("batched: %d problems, FP32 state" % len(pars))
+    b = BatchSolver(pars, precision="fp32")
+    try:
+        dts, ns = b.prepare()
+        tile, bpc, times = autotune.tune(b, steps=2, use_cache=False)
+        log("batched: autotuner chose tile %s, %d workgroups/CU budget" % (tile, bpc))
+        b.prepare()
+        b.step(ns, use_graph=True)
+        tot_ms, ker_ms, launches, by, mcu, grid = b.timing()
+        log("batched: %d launches in %.3f s" % (launches, tot_ms / 1e3))
+        leg = {
+            "workload": "%d 2D P_N problems, sides 64..512, odd N 3..39, FP32 state" % len(pars),
+            "value": mcu / (tot_ms / 1e3),
+            "unit": "moment-cell updates/s",
+            "vs_fp64_plain": mcu / (tot_ms / 1e3) / fp64_plain_value if fp64_plain_value else None,
+            "mcu": mcu,
+            "seconds": tot_ms / 1e3,
+            "launches": launches,
+            "graph": True,
+            "persistent_grid": grid,
+            "achieved_GBs": by / (tot_ms / 1e3) / 1e9,
+            "frac": by / (tot_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "schedule": {"tile": list(tile), "blocks_per_cu": bpc,
+                         "candidates_ms": {"%s/%d" % (t, k): v for (t, k), v in times.items()}},
+        }
+        if check:
+            worst, worst_all, worst_i = 0.0, 0.0, -1
+            for i, (p, s32) in enumerate(zip(pars, b.solvers)):
+                s64 = HipSolver(p, precision="fp64")
+                try:
+                    dt = s64.prepare()
+                    s64.step(dt, int(ns[i]))
+                    e = fp32_error(s64, s32)
+                finally:
+                    s64.close()
+                if e["rel_l2_vs_fp64_max_over_moments"] > worst:
+                    worst = e["rel_l2_vs_fp64_max_over_moments"]
+                    worst_i = i
+                worst_all = max(worst_all, e["rel_l2_vs_fp64_all_moments"])
+            leg["rel_l2_vs_fp64_max_over_problems_and_moments"] = worst
+            leg["rel_l2_vs_fp64_all_moments_max_over_problems"] = worst_all
+            leg["worst_problem"] = {"index": worst_i, "side": int(pars[worst_i]["n"][1]),
+                                    "n_mom": int(pars[worst_i]["n_mom"])}
+    finally:
+        b.close()
+    return leg
+
+
 def cpu_baseline(prob, steps):
     """The FP64 oracle (C restatement of the reference step, OpenMP over
     components) timed on the host: `steps` full steps of the same workload."""
@@ -159,6 +221,10 @@ def main():
     ap.add_argument("--c3-steps", type=int, default=50,
                     help="steps of the 3D P7 periodic leg (0 skips it)")
     ap.add_argument("--c3-n", type=int, default=200, help="grid side of the 3D leg")
+    ap.add_argument("--batch", type=int, default=64,
+                    help="problems of the batched leg (0 skips it)")
+    ap.add_argument("--no-batch-check", action="store_true",
+                    help="skip the FP64 solve of every batched problem")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the time-stepping core is a single-GPU "
@@ -255,6 +321,9 @@ def main():
         v[0].close()
     if args.c3_steps > 0:
         out["gauss3d"] = gauss3d_leg(args.c3_n, 7, args.c3_steps, min(args.warmup, 5))
+    if args.batch > 0:
+        out["batched"] = batched_leg(out["value"] if main_key == "fp64" else None,
+                                     n_instances=args.batch, check=not args.no_batch_check)
 
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))

@@ -726,6 +726,9 @@ smp::SubstepArgs<T> make_args(smp_problem *h, int kind, double dt, int napp, int
         a.lds[e] = h->ts.lds[e];
     }
     a.slot_elems = h->ts.slot_elems;
+    a.lds0_shift = 0;
+    while ((1 << a.lds0_shift) < h->ts.lds[0]) ++a.lds0_shift;
+    a.inv_lds1 = 1.0f / (float)h->ts.lds[1];
     a.chunks = (h->ts.slot_elems * (h->elem / 4) + 63) / 64;
     a.n_tiles = h->n_tiles;
     const smp::Plan &p = h->plan[kind];

@@ -28,7 +28,9 @@ struct SubstepArgs {
     int32_t pitch0, ext1;     // storage index = i0 + pitch0*(i1 + ext1*i2)
     int32_t n[3], bc[3];      // per kernel axis
     int32_t tile[3], ntile[3], lds[3];
-    int32_t slot_elems;       // lds[0]*lds[1]*lds[2]
+    int32_t slot_elems;       // lds[0]*lds[1]*lds[2]; lds[0] is a power of two
+    int32_t lds0_shift;       // log2(lds[0])
+    float inv_lds1;           // 1/lds[1]
     int32_t chunks;           // DMA chunks (64 lanes x 4 bytes) per slot
     int32_t n_tiles;          // total tiles
     int32_t ngroups;

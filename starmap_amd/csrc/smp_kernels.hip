@@ -48,6 +48,17 @@ size_t substep_lds_overhead() { return 3 * (size_t)BLOB_SZ; }
 
 typedef __attribute__((address_space(3))) void *lds_ptr_t;
 
+// Pointers into device memory taken from argument structs that live in
+// memory (the batched kernel) are generic pointers to the compiler, and
+// generic accesses compile to flat instructions with conservative waits;
+// these casts state that they address global memory.
+#define SMP_GLOBAL __attribute__((address_space(1)))
+#define SMP_CONST __attribute__((address_space(4)))
+template <typename P>
+__device__ __forceinline__ SMP_GLOBAL P *glb(P *p) {
+    return (SMP_GLOBAL P *)p;
+}
+
 // One workgroup = one spatial tile of up to 256 cells.  Waves 0-3 own one
 // cell per thread and update the targets; waves 4-5 copy: source tiles and the
 // group records go from HBM straight into LDS (global_load_lds, no
@@ -60,13 +71,25 @@ typedef __attribute__((address_space(3))) void *lds_ptr_t;
 //
 // NAPP = 0 takes the number of half-step applications from a.napp (the
 // batched kernel, whose work units mix substeps of different kinds).
-template <typename T, int NT, int NAPP, bool GEN>
-__device__ __forceinline__ void substep_tile(const SubstepArgs<T> &a, const int tile,
-                                             unsigned char *smem_raw) {
+// AT: SubstepArgs<T> (kernel argument) or the same in the constant address
+// space (an entry of the batched kernel, read with scalar loads).
+template <typename T, int NT, int NAPP, bool GEN, typename AT>
+__device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsigned char *smem_raw) {
     unsigned char *ring = smem_raw;
     T *cfa = reinterpret_cast<T *>(smem_raw + 3 * BLOB_SZ);
     T *lds = cfa + a.ncf * 256;
     const int tid = threadIdx.x;
+    // arguments the update reads for every target, held in registers (a
+    // scalar reload inside the loop would also wait for the LDS reads in
+    // flight, which share its counter)
+    const double half_dt = __builtin_bit_cast(
+        double, ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(
+                     (int)(__builtin_bit_cast(uint64_t, (double)a.half_dt) >> 32))
+                 << 32) |
+                    (uint32_t)__builtin_amdgcn_readfirstlane(
+                        (int)(uint32_t)__builtin_bit_cast(uint64_t, (double)a.half_dt)));
+    const int fslot0 = __builtin_amdgcn_readfirstlane(a.fslot[0]);
+    const int fslot1 = __builtin_amdgcn_readfirstlane(a.fslot[1]);
     const int lane = tid & 63;
     const bool copier = tid >= 256;
     int b = tile;
@@ -119,10 +142,10 @@ __device__ __forceinline__ void substep_tile(const SubstepArgs<T> &a, const int 
             const int e = cc * EPC + lane / DPE;
             po[cc] = -1;
             if (cc < a.chunks && e < S) {
-                const int l0 = e % a.lds[0];
-                const int row = e / a.lds[0];
-                const int l1 = row % a.lds[1];
-                const int l2 = row / a.lds[1];
+                const int l0 = e & (a.lds[0] - 1);
+                const int row = e >> a.lds0_shift;
+                const int l2 = (int)(((float)row + 0.5f) * a.inv_lds1);   // exact for rows < 2^16
+                const int l1 = row - l2 * a.lds[1];
                 const int x0 = wrap_or_clamp(o0 + l0, a.n[0], a.bc[0], a.n[0] + a.bc[0]);
                 const int x1 = wrap_or_clamp(o1 + l1, a.n[1], a.bc[1], a.n[1] + a.bc[1]);
                 const int x2 = wrap_or_clamp(o2 + l2, a.n[2], a.bc[2], a.n[2] + a.bc[2]);
@@ -145,7 +168,7 @@ __device__ __forceinline__ void substep_tile(const SubstepArgs<T> &a, const int 
         if (tid >= 320) return;
 #pragma unroll
         for (int k = 0; k < BLOB_SZ / 1024; ++k) {
-            const unsigned char *src = a.blobs + (size_t)gq * BLOB_SZ + k * 1024 + lane * 16;
+            SMP_GLOBAL const unsigned char *src = glb(a.blobs) + (size_t)gq * BLOB_SZ + k * 1024 + lane * 16;
             __builtin_amdgcn_global_load_lds(src, (lds_ptr_t)(ring + buf * BLOB_SZ + k * 1024), 16, 0,
                                              0);
         }
@@ -167,12 +190,12 @@ __device__ __forceinline__ void substep_tile(const SubstepArgs<T> &a, const int 
             const int comp = rec & 0xFFFFF;
             const int slot = (rec >> 20) & 0xFF;
             const int m = (rec >> 28) & 7;   // bit e: cell-centred on kernel axis e
-            const unsigned *cb = reinterpret_cast<const unsigned *>(a.U + (int64_t)comp * a.slab);
+            SMP_GLOBAL const unsigned *cb = glb(reinterpret_cast<const unsigned *>(a.U + (int64_t)comp * a.slab));
             unsigned char *dst = reinterpret_cast<unsigned char *>(lds + slot * S);
             if (low_inner) {
                 // every shifted coordinate is x-1: one uniform base shift
-                const unsigned *cbs = cb - ((m & 1) ? st[0] : 0) - ((m & 2) ? st[1] : 0) -
-                                      ((m & 4) ? st[2] : 0);
+                SMP_GLOBAL const unsigned *cbs = cb - ((m & 1) ? st[0] : 0) - ((m & 2) ? st[1] : 0) -
+                                                 ((m & 4) ? st[2] : 0);
 #pragma unroll
                 for (int cc = 0; cc < CPS_MAX; ++cc) {
                     // lanes past the end of the slot copy nothing
@@ -202,12 +225,13 @@ __device__ __forceinline__ void substep_tile(const SubstepArgs<T> &a, const int 
     // number of memory operations per group is fixed and the compiler can
     // wait for a prefetched target value with an exact count instead of
     // draining every load and store in flight.
-    const char *ug_ld = reinterpret_cast<const char *>(a.U) + (in_store ? gidx : 0) * sizeof(T);
-    char *ug_st = reinterpret_cast<char *>(a.U) + gidx * sizeof(T);
+    SMP_GLOBAL const char *ug_ld = glb(reinterpret_cast<const char *>(a.U)) + (in_store ? gidx : 0) * sizeof(T);
+    SMP_GLOBAL char *ug_st = glb(reinterpret_cast<char *>(a.U)) + gidx * sizeof(T);
+    SMP_GLOBAL T *sink = glb(a.sink);
     auto load_targets = [&](int buf, T *u) {
         const TgtRec *r = recs(buf);
 #pragma unroll
-        for (int i = 0; i < GMAX; ++i) u[i] = *reinterpret_cast<const T *>(ug_ld + r[i].uoff);
+        for (int i = 0; i < GMAX; ++i) u[i] = *reinterpret_cast<SMP_GLOBAL const T *>(ug_ld + r[i].uoff);
     };
     // byte address of this thread's cell in the first source slot; term
     // offsets are byte offsets from the start of LDS
@@ -221,7 +245,7 @@ __device__ __forceinline__ void substep_tile(const SubstepArgs<T> &a, const int 
     auto coef = [&](int off, const void *p, double s) -> A {
         const A staged = (A)cfa[(off > 0 ? off : 0) + tid];
         A v = off >= 0 ? staged : (A)s;
-        if (GEN && off == -2) v = in_store ? (A)reinterpret_cast<const T *>(p)[gidx] : (A)0;
+        if (GEN && off == -2) v = in_store ? (A)glb(reinterpret_cast<const T *>(p))[gidx] : (A)0;
         return v;
     };
     // Every group holds GMAX target records; unused ones are zero records
@@ -250,21 +274,20 @@ __device__ __forceinline__ void substep_tile(const SubstepArgs<T> &a, const int 
             const A wq = w + coef(h1.x, r.Q_p, 0.0);
             A u = (A)u_in[i];
 #pragma unroll
-            for (int app = 0; app < (NAPP ? NAPP : 2); ++app) {
-                if (NAPP == 0 && app >= a.napp) break;
+            for (int app = 0; app < NAPP; ++app) {
                 // U + dt/2*(W+Q - s.*U).*E   (starmap_solver.m:352-354)
-                u = u + (a.half_dt * (wq - sig * u)) * E;
-                const bool s1 = a.fslot[app] != 0;
+                u = u + (half_dt * (wq - sig * u)) * E;
+                const bool s1 = (app == 0 ? fslot0 : fslot1) != 0;
                 A f = s1 ? r.F_s[1] : r.F_s[0];   // 1 for targets without a filter
                 if (GEN && h0.y) {
                     const void *fp = s1 ? r.F[1] : r.F[0];
-                    if (fp && act) f = (A)reinterpret_cast<const T *>(fp)[gidx];
+                    if (fp && act) f = (A)glb(reinterpret_cast<const T *>(fp))[gidx];
                 }
                 // the state after every half-step is a value of the state
                 // precision, as if stored: fused and separate launches agree
                 u = (A)(T)(u * f);
             }
-            T *dst = act ? reinterpret_cast<T *>(ug_st + r.uoff) : a.sink + tid;
+            SMP_GLOBAL T *dst = act ? reinterpret_cast<SMP_GLOBAL T *>(ug_st + r.uoff) : sink + tid;
             *dst = (T)u;
         }
     };
@@ -276,8 +299,15 @@ __device__ __forceinline__ void substep_tile(const SubstepArgs<T> &a, const int 
         if (ng > 1) blob_dma(1, 1);
         __builtin_amdgcn_s_waitcnt(0);
     } else {
-        for (int c = 0; c < a.ncf; ++c)
-            cfa[c * 256 + tid] = in_store ? reinterpret_cast<const T *>(a.cfields[c])[gidx] : (T)0;
+        // every field load in flight before the first LDS write
+        T cv[MAX_CFIELDS];
+#pragma unroll
+        for (int c = 0; c < MAX_CFIELDS; ++c)
+            if (c < a.ncf)
+                cv[c] = glb(reinterpret_cast<const T *>(a.cfields[c]))[in_store ? gidx : 0];
+#pragma unroll
+        for (int c = 0; c < MAX_CFIELDS; ++c)
+            if (c < a.ncf) cfa[c * 256 + tid] = in_store ? cv[c] : (T)0;
         for (int e = tid; e < S; e += 256) lds[e] = (T)0;   // slot 0: the zero tile
     }
     __syncthreads();
@@ -329,9 +359,12 @@ __global__ __launch_bounds__(KTHREADS) void batch_kernel(const BatchLaunch<T> *b
     __shared__ int s_unit;
     const int n_units = bl->n_units;
     const int n_entries = bl->n_entries;
-    const BatchEntry<T> *ent = bl->entries;
+    // the entries are not written during the launch: scalar loads
+    const SMP_CONST BatchEntry<T> *ent = (const SMP_CONST BatchEntry<T> *)bl->entries;
     for (;;) {
-        if (threadIdx.x == 0) s_unit = atomicAdd(bl->counter, 1);
+        if (threadIdx.x == 0)
+            s_unit = __hip_atomic_fetch_add(glb(bl->counter), 1, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         const int u = __builtin_amdgcn_readfirstlane(s_unit);
         __syncthreads();
@@ -345,7 +378,22 @@ __global__ __launch_bounds__(KTHREADS) void batch_kernel(const BatchLaunch<T> *b
             else
                 hi = mid - 1;
         }
-        substep_tile<T, NT, 0, GEN>(ent[lo].a, u - ent[lo].unit0, smem_raw);
+        // the entry's arguments, copied into registers as the plain
+        // kernel's arguments are (reading them from memory inside the loops
+        // costs the update its overlap of LDS reads)
+        SubstepArgs<T> a;
+        static_assert(sizeof(SubstepArgs<T>) % 4 == 0, "argument dwords");
+        const SMP_CONST int *src = (const SMP_CONST int *)&ent[lo].a;
+        int *dst = reinterpret_cast<int *>(&a);
+#pragma unroll
+        for (int k = 0; k < (int)(sizeof(SubstepArgs<T>) / 4); ++k)
+            dst[k] = __builtin_amdgcn_readfirstlane(src[k]);
+        // one or two half-step applications, each compiled with its count
+        // fixed (a runtime count costs the update its instruction schedule)
+        if (a.napp == 2)
+            substep_tile<T, NT, 2, GEN>(a, u - ent[lo].unit0, smem_raw);
+        else
+            substep_tile<T, NT, 1, GEN>(a, u - ent[lo].unit0, smem_raw);
     }
 }
 
@@ -392,29 +440,24 @@ hipError_t launch_substep(const SubstepArgs<T> &a, int nt, bool gen, int threads
 
 // (the dynamic LDS limit of batch_kernel is raised by batch_occupancy, which
 // runs before any launch and outside graph capture)
-template <typename T, int NT, bool GEN>
-static hipError_t launch_b(const BatchLaunch<T> *bl, int grid, size_t lds_bytes, hipStream_t s) {
-    hipLaunchKernelGGL((batch_kernel<T, NT, GEN>), dim3(grid), dim3(KTHREADS), lds_bytes, s, bl);
-    return hipGetLastError();
+template <typename T>
+static const void *batch_fn(int nt, bool gen) {
+    if (nt <= 8) return gen ? (const void *)batch_kernel<T, 8, true> : (const void *)batch_kernel<T, 8, false>;
+    return gen ? (const void *)batch_kernel<T, MAXT, true> : (const void *)batch_kernel<T, MAXT, false>;
 }
 
 template <typename T>
 hipError_t launch_batch(const BatchLaunch<T> *bl, int nt, bool gen, int grid, size_t lds_bytes,
                         hipStream_t s) {
     if (grid < 1 || lds_bytes > 160 * 1024 - 64) return hipErrorInvalidValue;
-    if (nt <= 8) return gen ? launch_b<T, 8, true>(bl, grid, lds_bytes, s)
-                            : launch_b<T, 8, false>(bl, grid, lds_bytes, s);
-    return gen ? launch_b<T, MAXT, true>(bl, grid, lds_bytes, s)
-               : launch_b<T, MAXT, false>(bl, grid, lds_bytes, s);
+    const void *f = batch_fn<T>(nt, gen);
+    void *args[] = {(void *)&bl};
+    return hipLaunchKernel(f, dim3(grid), dim3(KTHREADS), args, lds_bytes, s);
 }
 
 template <typename T>
 hipError_t batch_occupancy(int nt, bool gen, size_t lds_bytes, int *blocks) {
-    const void *f;
-    if (nt <= 8)
-        f = gen ? (const void *)batch_kernel<T, 8, true> : (const void *)batch_kernel<T, 8, false>;
-    else
-        f = gen ? (const void *)batch_kernel<T, MAXT, true> : (const void *)batch_kernel<T, MAXT, false>;
+    const void *f = batch_fn<T>(nt, gen);
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64);
     if (e != hipSuccess) return e;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, f, KTHREADS, lds_bytes);

@@ -94,7 +94,7 @@ std::string make_tile(const Geometry &g, int tx, int ty, int tz, TileShape &ts) 
     }
     if (ts.cells() > 256) return "a tile holds at most 256 cells";
     // LDS rows of a power of two: the tile plus its one-cell halo
-    if (g.ndim > 1 && ((ts.tile[0] + 1) & ts.tile[0])) return "tile[0]+1 must be a power of two";
+    if ((ts.tile[0] + 1) & ts.tile[0]) return "tile[0]+1 must be a power of two";
     ts.threads = 256;
     for (int e = 0; e < 3; ++e) ts.lds[e] = e < g.ndim ? ts.tile[e] + 1 : 1;
     ts.slot_elems = ts.lds[0] * ts.lds[1] * ts.lds[2];
