@@ -210,6 +210,35 @@ int smp_batch_run(smp_batch *b, const double *dt, const int64_t *n_steps, int32_
 int smp_batch_timing(const smp_batch *b, double *total_ms, double *kernel_ms, int64_t *launches,
                      double *bytes, int64_t *mcu, int32_t *grid);
 
+/* ---- discrete adjoint (no counterpart in the reference) ----------------- */
+
+/* Zero the adjoint state lambda and the gradient accumulators. */
+int smp_adjoint_reset(smp_problem *h);
+
+/* Set component j of the adjoint state lambda (reference layout, as
+ * smp_set_state): the seed dJ/dU(T) of an objective J of the final state. */
+int smp_adjoint_seed(smp_problem *h, int32_t j, const double *src, int64_t count);
+
+/* Discrete adjoint of n_steps time steps of size dt from the current state
+ * (the arithmetic of smp_step(h, dt, n_steps), starmap_solver.m:307-421):
+ * runs the steps forward -- the state ends at the final time, as after
+ * smp_step -- keeping a copy of the state every checkpoint_every steps, then
+ * sweeps back from the seeded lambda(T), recomputing each segment from its
+ * checkpoint.  Afterwards lambda holds dJ/dU(0) and the accumulators hold
+ * dJ/dsigma_j for every component j at every point of its grid (FP64),
+ * where sigma_j is sigma_a for j = 1 and sigma_t of j's grid type and order
+ * otherwise.  Filters must be off or at 'substep' position and constant in
+ * space; sources must be fields. */
+int smp_adjoint(smp_problem *h, double dt, int64_t n_steps, int32_t checkpoint_every);
+
+/* Component j of the adjoint state (what = 0) or of dJ/dsigma_j (what = 1),
+ * reference layout. */
+int smp_adjoint_get(smp_problem *h, int32_t what, int32_t j, double *dst, int64_t count);
+
+/* Milliseconds of the last smp_adjoint: forward pass and reverse sweep
+ * (HIP events on the handle's stream). */
+int smp_adjoint_timing(const smp_problem *h, double *forward_ms, double *reverse_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,152 @@
+"""Gradients of a functional of the final state with respect to the
+materials, by the discrete adjoint of the time stepper (``smp_adjoint``).
+
+``Adjoint(par)`` wraps a device handle.  ``run(seed, n_steps, every)`` runs
+``n_steps`` steps forward from the initial state (keeping a state copy every
+``every`` steps) and back, and returns dJ/dsigma_j for every component;
+``material_gradients`` sums them to dJ/dsigma_a and dJ/dsigma_s0 on each
+staggered grid type (sigma_t = sigma_a + sigma_s0 - sigma_sm(m) for j > 1,
+sigma_a for j = 1, starmap_solver.m:214-252), and ``cell_gradients`` to
+per-cell values through the map from grid points to owning cells that the
+material functions use.
+
+``StarmapObjective`` is a ``torch.autograd.Function`` over the same library:
+per-cell sigma_a and sigma_s0 in, the objective out, gradients from the
+adjoint in backward.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .batch import count_steps
+from .init import capargs, starmap_init
+from .solver import HipSolver
+
+
+class Adjoint(HipSolver):
+    """A HipSolver with the adjoint entry points."""
+
+    def seed(self, j, arr):
+        a = np.asfortranarray(np.broadcast_to(np.asarray(arr, dtype=np.float64), self.shape(j)))
+        flat = np.ascontiguousarray(a.ravel(order="F"))
+        _lib.check(_lib.lib().smp_adjoint_seed(self.handle, j, flat.ctypes.data, flat.size))
+
+    def reset(self):
+        _lib.check(_lib.lib().smp_adjoint_reset(self.handle))
+
+    def _get(self, what, j):
+        shp = self.shape(j)
+        out = np.empty(int(np.prod(shp)), dtype=np.float64)
+        _lib.check(_lib.lib().smp_adjoint_get(self.handle, what, j, out.ctypes.data, out.size))
+        return out.reshape(shp, order="F")
+
+    def lam(self, j):
+        """Component j of the adjoint state (dJ/dU_j at the start)."""
+        return self._get(0, j)
+
+    def dsigma(self, j):
+        """dJ/dsigma_j on the grid of component j."""
+        return self._get(1, j)
+
+    def adjoint_timing(self):
+        f, r = ctypes.c_double(), ctypes.c_double()
+        _lib.check(_lib.lib().smp_adjoint_timing(self.handle, ctypes.byref(f), ctypes.byref(r)))
+        return f.value, r.value
+
+    def run(self, seeds, n_steps=None, every=50):
+        """Initial state and coefficients (prepare), then the adjoint of
+        n_steps steps (default: to t_final) seeded with {j: dJ/dU_j(T)}.
+        Returns (dt, n_steps)."""
+        dt = self.prepare()
+        if n_steps is None:
+            n_steps = count_steps(dt, self.par["tfinal"])
+        self.reset()
+        for j, arr in seeds.items():
+            self.seed(j, arr)
+        _lib.check(_lib.lib().smp_adjoint(self.handle, float(dt), int(n_steps), int(every)))
+        self._used = True
+        return dt, n_steps
+
+    def material_gradients(self):
+        """{grid type: (dJ/dsigma_a, dJ/dsigma_s0)} on each staggered grid."""
+        out = {}
+        for j in range(self.n_sys):
+            g = self.dsigma(j)
+            key = self.gkey[j]
+            ga, gs = out.get(key, (0.0, 0.0))
+            ga = ga + g
+            if j > 0:
+                gs = gs + g
+            out[key] = (ga, gs)
+        return out
+
+    def cell_gradients(self, owner_par, n_cells):
+        """Per-cell dJ/dsigma_a and dJ/dsigma_s0 (length n_cells), summing
+        every grid point into its owning cell; owner_par is a problem dict
+        whose sigma_a returns the owning cell index of a point."""
+        own = starmap_init(owner_par)["sigma_a"]
+        ga = np.zeros(n_cells)
+        gs = np.zeros(n_cells)
+        for key, (a, s) in self.material_gradients().items():
+            X, Y, Z = self.grid(key)
+            idx = np.broadcast_to(np.asarray(capargs(own, X, Y, Z)), np.shape(a)).astype(np.int64)
+            ga += np.bincount(idx.ravel(order="F"), weights=np.ravel(a, order="F"), minlength=n_cells)
+            if np.ndim(s):
+                gs += np.bincount(idx.ravel(order="F"), weights=np.ravel(s, order="F"),
+                                  minlength=n_cells)
+        return ga, gs
+
+
+def strip_objective(solver, xmin=0.8):
+    """J = h^2 * sum of U1(T) over the cell centres with x > xmin (2D) and
+    its seed dJ/dU1."""
+    X, Y, Z = solver.grid(solver.gkey[0])
+    h2 = float(solver.h[1] * solver.h[2])
+    mask = (np.asarray(Y) > xmin).astype(np.float64) * h2
+    mask = np.broadcast_to(mask, solver.shape(0))
+    return mask
+
+
+def objective_value(solver, mask):
+    return float(np.sum(solver.get_state(0) * mask))
+
+
+try:
+    import torch
+
+    class StarmapObjective(torch.autograd.Function):
+        """J(sigma_a, sigma_s0) for per-cell n x n tensors: the strip
+        objective of a problem built by make_prob(sigma_a, sigma_s0) (a
+        numpy -> prob dict function), forward by the time stepper, gradient
+        by its discrete adjoint (FP64 accumulation)."""
+
+        @staticmethod
+        def forward(ctx, sigma_a, sigma_s0, make_prob, precision="fp32", every=50):
+            sa = sigma_a.detach().cpu().double().numpy()
+            ss = sigma_s0.detach().cpu().double().numpy()
+            prob = make_prob(sa, ss)
+            par = starmap_init(prob)
+            s = Adjoint(par, precision=precision)
+            try:
+                mask = strip_objective(s)
+                s.run({0: mask}, every=every)
+                J = objective_value(s, mask)
+                n = sa.shape[0]
+                owner_prob = dict(prob, sigma_a=(lambda x, y: prob["_owner"](x, y).astype(np.float64)))
+                ga, gs = s.cell_gradients(owner_prob, n * n)
+            finally:
+                s.close()
+            ctx.save_for_backward(torch.from_numpy(ga.reshape(sa.shape)),
+                                  torch.from_numpy(gs.reshape(ss.shape)))
+            ctx.dev, ctx.dtype = sigma_a.device, sigma_a.dtype
+            return torch.tensor(J, dtype=torch.float64, device=sigma_a.device)
+
+        @staticmethod
+        def backward(ctx, gout):
+            ga, gs = ctx.saved_tensors
+            g = gout.detach().double().cpu()
+            return ((ga * g).to(ctx.dev, ctx.dtype), (gs * g).to(ctx.dev, ctx.dtype),
+                    None, None, None)
+except ImportError:          # torch is optional for the gradient itself
+    StarmapObjective = None

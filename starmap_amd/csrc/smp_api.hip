@@ -103,6 +103,13 @@ struct smp_problem {
     void *sink = nullptr;         // 256 elements: stores of lanes without a target cell
     std::vector<hipEvent_t> events;
     std::vector<void *> ckpt;     // device copies of the whole state
+    // discrete adjoint: lambda, mu (state layout), dJ/dsigma per component
+    // (FP64), tables of the adjoint kernels, segment checkpoints and the
+    // states recorded while a segment is recomputed
+    void *lam = nullptr, *mu = nullptr, *G = nullptr, *work = nullptr;
+    void *adj_comp = nullptr, *adj_rows = nullptr, *adj_cols = nullptr, *adj_list = nullptr;
+    std::vector<void *> adj_seg, adj_rec;
+    double adj_fwd_ms = 0, adj_rev_ms = 0;
     double last_total_ms = 0, last_substep_ms = 0, last_bytes = 0;
     int64_t last_launches = 0;
     int64_t dev_bytes = 0;
@@ -705,10 +712,11 @@ int refresh_coefs(smp_problem *h) {
 }
 
 template <typename T>
-smp::SubstepArgs<T> make_args(smp_problem *h, int kind, double dt, int napp, int fs0, int fs1) {
+smp::SubstepArgs<T> make_args(smp_problem *h, int kind, double dt, int napp, int fs0, int fs1,
+                              void *U = nullptr) {
     smp::SubstepArgs<T> a;
     std::memset(&a, 0, sizeof(a));
-    a.U = (T *)h->U;
+    a.U = (T *)(U ? U : h->U);
     a.sink = (T *)h->sink;
     a.slab = h->slab;
     a.pitch0 = h->pitch0;
@@ -854,6 +862,247 @@ double substep_bytes(const smp_problem *h, int kind) {
     const double cells = (double)h->geo.n[0] * h->geo.n[1] * h->geo.n[2];
     const double ne = (double)h->geo.even.size(), no = (double)h->geo.odd.size();
     return (double)h->elem * cells * (kind == 1 ? no + 2 * ne : ne + 2 * no);
+}
+
+// ---------------------------------------------------------------------------
+// Discrete adjoint of n time steps
+
+// device tables of the adjoint kernels (coefficients as the forward uses
+// them at the current dt)
+template <typename T>
+int adjoint_tables(smp_problem *h) {
+    const smp::Geometry &g = h->geo;
+    const int n = g.n_sys;
+    int kaxis_of[3] = {-1, -1, -1};
+    for (int e = 0; e < g.ndim; ++e) kaxis_of[g.axis[e]] = e;
+    std::vector<smp::AdjComp<T>> comp(n);
+    std::vector<smp::AdjTerm> rows, cols;
+    std::vector<std::vector<smp::AdjTerm>> by_src(n);
+    for (int j = 0; j < n; ++j) {
+        smp::AdjComp<T> &c = comp[j];
+        std::memset(&c, 0, sizeof(c));
+        const Field *sig, *E;
+        if (j == 0) {
+            sig = &h->sA1;
+            E = &h->EA;
+        } else {
+            Key k(g.gt[3 * j], g.gt[3 * j + 1], g.gt[3 * j + 2], h->mom_order[j], 0);
+            auto it = h->sT.find(k);
+            if (it == h->sT.end() || !it->second.set)
+                return fail(SMP_E_STATE, "sigma_t not set for component " + std::to_string(j));
+            sig = &it->second;
+            E = &h->ET[k];
+        }
+        c.sig = (const T *)sig->dev;
+        c.sig_s = sig->scalar;
+        c.E = (const T *)E->dev;
+        c.E_s = E->scalar;
+        c.Q = h->Q[j].set ? (const T *)h->Q[j].dev : nullptr;
+        if (h->Q[j].set && !h->Q[j].dev && h->Q[j].scalar != 0.0)
+            return fail(SMP_E_INVALID, "the adjoint needs sources as fields");
+        c.F[0] = c.F[1] = 1.0;
+        if (j > 0 && h->f_position == SMP_FILTER_SUBSTEP) {
+            for (int sl = 0; sl < 2; ++sl) {
+                Key k(g.gt[3 * j], g.gt[3 * j + 1], g.gt[3 * j + 2], h->f_order[j], sl + 1);
+                auto it = h->F.find(k);
+                if (it == h->F.end() || !it->second.set)
+                    return fail(SMP_E_STATE, "substep filter not set for component " + std::to_string(j));
+                if (it->second.dev)
+                    return fail(SMP_E_INVALID, "the adjoint supports filters that are constant in space");
+                c.F[sl] = it->second.scalar;
+            }
+        }
+        for (int e = 0; e < 3; ++e) {
+            c.ext[e] = e < g.ndim ? g.ext_of(j, g.axis[e]) : 1;
+            c.cell[e] = e < g.ndim && g.gt[3 * j + g.axis[e]] == 1;
+        }
+        c.r0 = (int32_t)rows.size();
+        for (int d = 0; d < 3; ++d) {
+            const smp::Csr &M = g.M[d];
+            if (M.rp.empty()) continue;
+            for (int q = M.rp[j]; q < M.rp[j + 1]; ++q) {
+                smp::AdjTerm t;
+                t.k = M.ci[q];
+                t.axis = kaxis_of[d];
+                t.coef = -M.cv[q] / g.h[d];
+                rows.push_back(t);
+                smp::AdjTerm u = t;
+                u.k = j;
+                by_src[M.ci[q]].push_back(u);
+            }
+        }
+        c.r1 = (int32_t)rows.size();
+    }
+    for (int k = 0; k < n; ++k) {
+        comp[k].c0 = (int32_t)cols.size();
+        cols.insert(cols.end(), by_src[k].begin(), by_src[k].end());
+        comp[k].c1 = (int32_t)cols.size();
+    }
+    std::vector<int32_t> list(g.odd.begin(), g.odd.end());
+    list.insert(list.end(), g.even.begin(), g.even.end());
+    for (void **p : {&h->adj_comp, &h->adj_rows, &h->adj_cols, &h->adj_list}) {
+        h->dfree(*p);
+        *p = nullptr;
+    }
+    int rc = h->dalloc(&h->adj_comp, sizeof(smp::AdjComp<T>) * n);
+    if (!rc) rc = h->dalloc(&h->adj_rows, sizeof(smp::AdjTerm) * std::max<size_t>(1, rows.size()));
+    if (!rc) rc = h->dalloc(&h->adj_cols, sizeof(smp::AdjTerm) * std::max<size_t>(1, cols.size()));
+    if (!rc) rc = h->dalloc(&h->adj_list, sizeof(int32_t) * std::max<size_t>(1, list.size()));
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(h->adj_comp, comp.data(), sizeof(smp::AdjComp<T>) * n, hipMemcpyHostToDevice));
+    if (!rows.empty())
+        HIP_TRY(hipMemcpy(h->adj_rows, rows.data(), sizeof(smp::AdjTerm) * rows.size(),
+                          hipMemcpyHostToDevice));
+    if (!cols.empty())
+        HIP_TRY(hipMemcpy(h->adj_cols, cols.data(), sizeof(smp::AdjTerm) * cols.size(),
+                          hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->adj_list, list.data(), sizeof(int32_t) * list.size(), hipMemcpyHostToDevice));
+    return SMP_OK;
+}
+
+int adjoint_buffers(smp_problem *h) {
+    const size_t st = (size_t)h->slab * h->elem * h->geo.n_sys;
+    for (void **p : {&h->lam, &h->mu, &h->work}) {
+        if (!*p) {
+            int rc = h->dalloc(p, st);
+            if (rc) return rc;
+            HIP_TRY(hipMemsetAsync(*p, 0, st, h->stream));
+        }
+    }
+    if (!h->G) {
+        int rc = h->dalloc(&h->G, (size_t)h->slab * 8 * h->geo.n_sys);
+        if (rc) return rc;
+        HIP_TRY(hipMemsetAsync(h->G, 0, (size_t)h->slab * 8 * h->geo.n_sys, h->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return SMP_OK;
+}
+
+template <typename T>
+int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
+    if (h->f_position == SMP_FILTER_FULL)
+        return fail(SMP_E_INVALID, "the adjoint supports no filter or the substep filter");
+    int rc = adjoint_buffers(h);
+    if (rc) return rc;
+    // the forward coefficients at this dt
+    if (h->decay_dirty || !(h->dt_decay == dt)) {
+        rc = refresh_decay<T>(h, dt);
+        if (rc) return rc;
+    }
+    if (h->coef_dirty) {
+        rc = refresh_coefs<T>(h);
+        if (rc) return rc;
+    }
+    rc = adjoint_tables<T>(h);
+    if (rc) return rc;
+    const size_t st = (size_t)h->slab * h->elem * h->geo.n_sys;
+    const int64_t n_seg = (n_steps + every - 1) / every;
+    // checkpoint buffers: one state per segment start, two per step of a
+    // segment (before substep 1, after substep 2)
+    while ((int64_t)h->adj_seg.size() < n_seg) {
+        void *p = nullptr;
+        rc = h->dalloc(&p, st);
+        if (rc) return rc;
+        h->adj_seg.push_back(p);
+    }
+    while ((int64_t)h->adj_rec.size() < 2 * (int64_t)every) {
+        void *p = nullptr;
+        rc = h->dalloc(&p, st);
+        if (rc) return rc;
+        h->adj_rec.push_back(p);
+    }
+    hipEvent_t e0, e1, e2;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventCreate(&e2));
+    HIP_TRY(hipEventRecord(e0, h->stream));
+    // forward pass: segment checkpoints, the state ends at t = n_steps*dt
+    for (int64_t sg = 0; sg < n_seg; ++sg) {
+        HIP_TRY(hipMemcpyAsync(h->adj_seg[sg], h->U, st, hipMemcpyDeviceToDevice, h->stream));
+        const int64_t ns = std::min<int64_t>(every, n_steps - sg * every);
+        rc = run_steps<T>(h, dt, ns);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipEventRecord(e1, h->stream));
+    const int nl_odd = (int)h->geo.odd.size(), nl_even = (int)h->geo.even.size();
+    const int32_t *list_odd = (const int32_t *)h->adj_list;
+    const int32_t *list_even = list_odd + nl_odd;
+    auto adj_args = [&](const void *U, const void *U0, const int32_t *list, int nl, int napp, int f0,
+                        int f1) {
+        smp::AdjArgs<T> a;
+        std::memset(&a, 0, sizeof(a));
+        a.comp = (const smp::AdjComp<T> *)h->adj_comp;
+        a.rows = (const smp::AdjTerm *)h->adj_rows;
+        a.cols = (const smp::AdjTerm *)h->adj_cols;
+        a.list = list;
+        a.n_list = nl;
+        a.U = (const T *)U;
+        a.U0 = (const T *)U0;
+        a.lam = (T *)h->lam;
+        a.mu = (T *)h->mu;
+        a.G = (double *)h->G;
+        a.slab = h->slab;
+        a.pitch0 = h->pitch0;
+        a.ext1 = h->E[1];
+        for (int e = 0; e < 3; ++e) {
+            a.E[e] = h->E[e];
+            a.n[e] = e < h->geo.ndim ? h->geo.n[h->geo.axis[e]] : 1;
+            a.bc[e] = e < h->geo.ndim ? h->geo.bc[h->geo.axis[e]] : 0;
+        }
+        a.half_dt = dt / 2;
+        a.napp = napp;
+        a.fslot[0] = f0;
+        a.fslot[1] = f1;
+        return a;
+    };
+    // reverse sweep, one segment at a time: recompute it from its checkpoint
+    // recording the states each adjoint substep needs, then step back
+    for (int64_t sg = n_seg - 1; sg >= 0; --sg) {
+        const int64_t ns = std::min<int64_t>(every, n_steps - sg * every);
+        HIP_TRY(hipMemcpyAsync(h->work, h->adj_seg[sg], st, hipMemcpyDeviceToDevice, h->stream));
+        const smp::SubstepArgs<T> s1 = make_args<T>(h, 0, dt, 1, 0, 0, h->work);
+        const smp::SubstepArgs<T> s2 = make_args<T>(h, 1, dt, 2, 0, 1, h->work);
+        const smp::SubstepArgs<T> s3 = make_args<T>(h, 0, dt, 1, 1, 1, h->work);
+        for (int64_t s = 0; s < ns; ++s) {
+            HIP_TRY(hipMemcpyAsync(h->adj_rec[2 * s], h->work, st, hipMemcpyDeviceToDevice, h->stream));
+            HIP_TRY(smp::launch_substep<T>(s1, h->plan[0].max_terms, h->gen[0], smp::KTHREADS,
+                                           h->lds_bytes[0], h->stream));
+            HIP_TRY(smp::launch_substep<T>(s2, h->plan[1].max_terms, h->gen[1], smp::KTHREADS,
+                                           h->lds_bytes[1], h->stream));
+            HIP_TRY(hipMemcpyAsync(h->adj_rec[2 * s + 1], h->work, st, hipMemcpyDeviceToDevice,
+                                   h->stream));
+            if (s + 1 < ns)
+                HIP_TRY(smp::launch_substep<T>(s3, h->plan[0].max_terms, h->gen[0], smp::KTHREADS,
+                                               h->lds_bytes[0], h->stream));
+        }
+        for (int64_t s = ns - 1; s >= 0; --s) {
+            const void *S0 = h->adj_rec[2 * s], *S1 = h->adj_rec[2 * s + 1];
+            // substep 3: odd from S1's odd, sources S1's even; filter slot k=2
+            HIP_TRY(smp::launch_adj_update<T>(adj_args(S1, S1, list_odd, nl_odd, 1, 1, 1), h->stream));
+            HIP_TRY(smp::launch_adj_transpose<T>(adj_args(S1, S1, list_even, nl_even, 1, 1, 1),
+                                                 h->stream));
+            // substep 2: even from S0's even, sources S1's odd; two
+            // applications, slots k=1 then k=2
+            HIP_TRY(smp::launch_adj_update<T>(adj_args(S1, S0, list_even, nl_even, 2, 0, 1), h->stream));
+            HIP_TRY(smp::launch_adj_transpose<T>(adj_args(S1, S0, list_odd, nl_odd, 2, 0, 1),
+                                                 h->stream));
+            // substep 1: odd from S0's odd, sources S0's even; slot k=1
+            HIP_TRY(smp::launch_adj_update<T>(adj_args(S0, S0, list_odd, nl_odd, 1, 0, 0), h->stream));
+            HIP_TRY(smp::launch_adj_transpose<T>(adj_args(S0, S0, list_even, nl_even, 1, 0, 0),
+                                                 h->stream));
+        }
+    }
+    HIP_TRY(hipEventRecord(e2, h->stream));
+    HIP_TRY(hipEventSynchronize(e2));
+    float f = 0, r = 0;
+    HIP_TRY(hipEventElapsedTime(&f, e0, e1));
+    HIP_TRY(hipEventElapsedTime(&r, e1, e2));
+    h->adj_fwd_ms = f;
+    h->adj_rev_ms = r;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    return SMP_OK;
 }
 
 }  // namespace
@@ -1380,5 +1629,89 @@ int smp_batch_timing(const smp_batch *b, double *total_ms, double *kernel_ms, in
     return SMP_OK;
 }
 
+int smp_adjoint_reset(smp_problem *h) {
+    if (!h) return fail(SMP_E_INVALID, "null handle");
+    REQUIRE_DEVICE(h);
+    HIP_TRY(hipSetDevice(h->device));
+    int rc = adjoint_buffers(h);
+    if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(h->lam, 0, (size_t)h->slab * h->elem * h->geo.n_sys, h->stream));
+    HIP_TRY(hipMemsetAsync(h->G, 0, (size_t)h->slab * 8 * h->geo.n_sys, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return SMP_OK;
+}
+
+int smp_adjoint_seed(smp_problem *h, int32_t j, const double *src, int64_t count) {
+    if (!h || !src) return fail(SMP_E_INVALID, "null argument");
+    REQUIRE_DEVICE(h);
+    if (j < 0 || j >= h->geo.n_sys) return fail(SMP_E_INVALID, "component out of range");
+    if (count != h->numel(j)) return fail(SMP_E_INVALID, "count does not match the component shape");
+    HIP_TRY(hipSetDevice(h->device));
+    int rc = adjoint_buffers(h);
+    if (rc) return rc;
+    const int gt[3] = {h->geo.gt[3 * j], h->geo.gt[3 * j + 1], h->geo.gt[3 * j + 2]};
+    char *dst = (char *)h->lam + (size_t)j * h->slab * h->elem;
+    if (h->precision == SMP_FP64) {
+        std::vector<double> tmp(h->slab);
+        h->pack<double>(gt, src, tmp.data());
+        HIP_TRY(hipMemcpy(dst, tmp.data(), (size_t)h->slab * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> tmp(h->slab);
+        h->pack<float>(gt, src, tmp.data());
+        HIP_TRY(hipMemcpy(dst, tmp.data(), (size_t)h->slab * 4, hipMemcpyHostToDevice));
+    }
+    return SMP_OK;
+}
+
+int smp_adjoint(smp_problem *h, double dt, int64_t n_steps, int32_t checkpoint_every) {
+    if (!h) return fail(SMP_E_INVALID, "null handle");
+    if (!(dt > 0) || !std::isfinite(dt)) return fail(SMP_E_INVALID, "dt must be positive and finite");
+    if (n_steps < 1) return fail(SMP_E_INVALID, "n_steps must be >= 1");
+    if (checkpoint_every < 1) return fail(SMP_E_INVALID, "checkpoint_every must be >= 1");
+    REQUIRE_DEVICE(h);
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    const int every = (int)std::min<int64_t>(checkpoint_every, n_steps);
+    return h->precision == SMP_FP64 ? run_adjoint<double>(h, dt, n_steps, every)
+                                    : run_adjoint<float>(h, dt, n_steps, every);
+}
+
+int smp_adjoint_get(smp_problem *h, int32_t what, int32_t j, double *dst, int64_t count) {
+    if (!h || !dst) return fail(SMP_E_INVALID, "null argument");
+    REQUIRE_DEVICE(h);
+    if (what != 0 && what != 1) return fail(SMP_E_INVALID, "what must be 0 (adjoint state) or 1 (gradient)");
+    if (j < 0 || j >= h->geo.n_sys) return fail(SMP_E_INVALID, "component out of range");
+    if (count != h->numel(j)) return fail(SMP_E_INVALID, "count does not match the component shape");
+    if (!h->lam) return fail(SMP_E_STATE, "no adjoint was run or seeded");
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    const int gt[3] = {h->geo.gt[3 * j], h->geo.gt[3 * j + 1], h->geo.gt[3 * j + 2]};
+    if (what == 1) {
+        std::vector<double> tmp(h->slab);
+        HIP_TRY(hipMemcpy(tmp.data(), (const char *)h->G + (size_t)j * h->slab * 8,
+                          (size_t)h->slab * 8, hipMemcpyDeviceToHost));
+        h->unpack<double>(gt, tmp.data(), dst);
+    } else if (h->precision == SMP_FP64) {
+        std::vector<double> tmp(h->slab);
+        HIP_TRY(hipMemcpy(tmp.data(), (const char *)h->lam + (size_t)j * h->slab * 8,
+                          (size_t)h->slab * 8, hipMemcpyDeviceToHost));
+        h->unpack<double>(gt, tmp.data(), dst);
+    } else {
+        std::vector<float> tmp(h->slab);
+        HIP_TRY(hipMemcpy(tmp.data(), (const char *)h->lam + (size_t)j * h->slab * 4,
+                          (size_t)h->slab * 4, hipMemcpyDeviceToHost));
+        h->unpack<float>(gt, tmp.data(), dst);
+    }
+    return SMP_OK;
+}
+
+int smp_adjoint_timing(const smp_problem *h, double *forward_ms, double *reverse_ms) {
+    if (!h) return fail(SMP_E_INVALID, "null handle");
+    if (forward_ms) *forward_ms = h->adj_fwd_ms;
+    if (reverse_ms) *reverse_ms = h->adj_rev_ms;
+    return SMP_OK;
+}
+
 }  // extern "C"
+
 

@@ -112,3 +112,66 @@ hipError_t launch_reduce(const T *U, const ReduceGeom &g, double *part, int bloc
 hipError_t launch_reduce_final(const double *part, int blocks, double *out3, hipStream_t s);
 
 }  // namespace smp
+
+namespace smp {
+
+// ---- discrete adjoint of the time step ------------------------------------
+// One coupling term of the adjoint tables: component `k` along kernel axis
+// `axis` with coefficient -M(j,k)/h (the forward kernel's coefficient).
+struct AdjTerm {
+    int32_t k;
+    int32_t axis;
+    double coef;
+};
+
+// Per component: its coefficients in the forward half-step, its extents and
+// grid types per kernel axis and its ranges in the term tables (terms by
+// target row and by source column).
+template <typename T>
+struct AdjComp {
+    const T *sig;             // sigma field (slab layout) or null: sig_s
+    const T *E;               // expm1div(-sigma dt/2) field or null: E_s
+    const T *Q;               // source field or null (no source)
+    double sig_s, E_s;
+    double F[2];              // filter factors of slots k=1,2 (1 without filter)
+    int32_t ext[3];
+    int32_t cell[3];          // 1 if cell-centred on the kernel axis
+    int32_t r0, r1;           // forward terms (this component as target)
+    int32_t c0, c1;           // transposed terms (this component as source)
+};
+
+template <typename T>
+struct AdjArgs {
+    const AdjComp<T> *comp;
+    const AdjTerm *rows;      // terms by target: k = source component
+    const AdjTerm *cols;      // terms by source: k = target component
+    const int32_t *list;      // components of the parity this launch updates
+    int32_t n_list;
+    const T *U;               // forward state during the substep (sources)
+    const T *U0;              // targets before the substep (rows kernel)
+    T *lam;                   // adjoint state
+    T *mu;                    // adjoint of the flux W of the targets
+    double *G;                // dJ/dsigma per component and point (FP64)
+    int64_t slab;
+    int32_t pitch0, ext1;
+    int32_t E[3];             // storage extents per kernel axis
+    int32_t n[3], bc[3];      // cells and boundary per kernel axis
+    double half_dt;
+    int32_t napp;
+    int32_t fslot[2];
+};
+
+// adjoint of the half-step applications of one substep for the targets in
+// a.list: mu = B*lambda', lambda = A*lambda', G += lambda'*(dA u + dB (W+Q))
+template <typename T>
+hipError_t launch_adj_update(const AdjArgs<T> &a, hipStream_t s);
+
+// lambda_k += sum_j coef_jk (D^T mu_j), k in a.list (the substep's sources)
+template <typename T>
+hipError_t launch_adj_transpose(const AdjArgs<T> &a, hipStream_t s);
+
+// dst[i] = src[i] for a contiguous range of elements
+template <typename T>
+hipError_t launch_copy(T *dst, const T *src, int64_t count, hipStream_t s);
+
+}  // namespace smp

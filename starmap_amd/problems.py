@@ -261,3 +261,74 @@ def batch_sweep(n_instances=64, seed=2025, tfinal=1.0, side_div=1, max_order=Non
         out.append(batch_instance(max(4, d["side"] // side_div), n_mom, d["f_eff"],
                                   d["sigma_a"], d["sigma_s0"], tfinal=tfinal))
     return out
+
+
+def gaussian_random_field(n, corr=0.2, std=0.5, seed=2026, length=2.0):
+    """Periodic Gaussian random field on the n x n cell grid of a square of
+    side `length`: white noise from numpy default_rng(seed) filtered in
+    Fourier space with a Gaussian kernel so that the covariance is
+    exp(-r^2 / (2 corr^2)), then shifted and scaled to mean 0 and std `std`."""
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal((n, n))
+    k = 2 * np.pi * np.fft.fftfreq(n, d=length / n)
+    kx, ky = np.meshgrid(k, k, indexing="ij")
+    # kernel of std corr/sqrt(2): the smoothed noise has covariance std corr
+    s = corr / np.sqrt(2.0)
+    f = np.real(np.fft.ifft2(np.fft.fft2(w) * np.exp(-(kx ** 2 + ky ** 2) * s ** 2 / 2)))
+    f = f - f.mean()
+    return f * (std / f.std())
+
+
+def cell_owner(n, half=1.0):
+    """Index (ix*n + iy) of the cell that owns a point (x, y) of [-half,half]^2
+    with the ceil() convention of starmap_2d_ex_lattice.m:50-53: a face on a
+    cell boundary belongs to the cell on its left/lower side, the boundary
+    at -half to the first cell."""
+    h = 2.0 * half / n
+
+    def owner(x, y):
+        ix = np.clip(np.ceil((np.asarray(x) + half) / h), 1, n).astype(np.int64) - 1
+        iy = np.clip(np.ceil((np.asarray(y) + half) / h), 1, n).astype(np.int64) - 1
+        return ix * n + iy
+    return owner
+
+
+def adjoint_grf(n=512, n_mom=15, steps=2000, corr=0.2, grf_std=0.5, seed=2026,
+                sigma_a=None, sigma_s0=None, src_std=0.05, closure="P"):
+    """2D P_N on [-1,1]^2 with extrapolation BC, per-cell sigma_a =
+    0.5*exp(Gaussian random field, correlation length corr, std grf_std)
+    and sigma_s0 = 1 (or the given n x n arrays), unit-mass Gaussian source
+    of std src_std at (-0.5, 0), zero initial state, no filter,
+    dt = 0.5*dx, `steps` steps (t_final = steps*dt, exact in binary for
+    n a power of two).  Materials on the staggered grids take the value of
+    the owning cell (cell_owner)."""
+    lam = legendre_max_root(n_mom + 1)
+    if sigma_a is None:
+        sigma_a = 0.5 * np.exp(gaussian_random_field(n, corr, grf_std, seed))
+    if sigma_s0 is None:
+        sigma_s0 = np.ones((n, n))
+    sa = np.asarray(sigma_a, dtype=np.float64).ravel()
+    ss = np.asarray(sigma_s0, dtype=np.float64).ravel()
+    owner = cell_owner(n)
+    dt = 0.5 * 2.0 / n
+
+    def source(x, y):
+        return np.exp(-((x + 0.5) ** 2 + y ** 2) / (2 * src_std ** 2)) / (2 * np.pi * src_std ** 2)
+
+    return {
+        "name": "Adjoint GRF 2D",
+        "closure": closure,
+        "n_mom": n_mom,
+        "sigma_a": (lambda x, y: sa[owner(x, y)]),
+        "sigma_s0": (lambda x, y: ss[owner(x, y)]),
+        "source": source,
+        "ax": [-1.0, 1.0, -1.0, 1.0],
+        "n": [n, n],
+        "bc": [1, 1],
+        "cn": lam,
+        "t_plot": [],
+        "tfinal": steps * dt,
+        "mom_output": 1,
+        "output": None,
+        "_owner": owner,
+    }

@@ -174,3 +174,19 @@ def test_batch_sweep_draws_are_seeded():
     assert {x["n_mom"] for x in a} <= set(P.BATCH_ORDERS)
     assert all(0 <= x["f_eff"] <= 20 for x in a)
     assert all(0.01 <= x["sigma_a"].min() and x["sigma_a"].max() <= 10 for x in a)
+
+
+def test_adjoint_entry_points_need_a_device():
+    from starmap_amd.adjoint import Adjoint
+    from starmap_amd import problems as P
+    from starmap_amd._lib import SmpError
+    s = Adjoint(I.starmap_init(P.adjoint_grf(n=8, n_mom=3, steps=4)), device=-1)
+    try:
+        L = _lib.lib()
+        assert L.smp_adjoint(s.handle, 0.1, 4, 2) == _lib.SMP_E_STATE
+        assert L.smp_adjoint(s.handle, -0.1, 4, 2) == _lib.SMP_E_INVALID
+        assert L.smp_adjoint(s.handle, 0.1, 4, 0) == _lib.SMP_E_INVALID
+        with pytest.raises(SmpError, match="without a device"):
+            s.reset()
+    finally:
+        s.close()

@@ -1,0 +1,179 @@
+"""GPU tests of the discrete adjoint (through the C ABI).
+
+* dJ/dU(0): J is affine in the initial state for fixed materials, so the
+  adjoint state at every probed point -- interior, edges and corners, both
+  parities, extrapolation and periodic boundaries -- equals a central
+  difference of the FP64 forward to rounding (1e-9 relative).  This pins
+  the transposed flux stencil and its boundary rules.
+* dJ/dsigma_a, dJ/dsigma_s0 per cell against FP64 central differences of
+  the forward (relative step 1e-6 of the cell value), 1e-5 relative to the
+  largest probed gradient; with and without the substep filter, with
+  checkpoints every few steps.
+* The same gradients against central differences of the oracle (the CPU
+  restatement of the reference step) on a small grid.
+* FP32 state with FP64 accumulation against the FP64 adjoint (1e-4).
+* The torch.autograd.Function returns the adjoint gradients.
+"""
+import numpy as np
+import pytest
+
+from oracle import starmap_oracle as so
+from starmap_amd import init as I
+from starmap_amd import problems as P
+from starmap_amd.adjoint import Adjoint, objective_value, strip_objective
+
+pytestmark = pytest.mark.gpu
+
+
+def small(n=12, n_mom=3, steps=9, bc=(1, 1), filt=False, seed=5, xmin=0.5):
+    rng = np.random.default_rng(seed)
+    sa = 0.5 * np.exp(rng.normal(0, 0.5, (n, n)))
+    ss = 1.0 + 0.3 * rng.random((n, n))
+    prob = P.adjoint_grf(n=n, n_mom=n_mom, steps=steps, sigma_a=sa, sigma_s0=ss, src_std=0.2)
+    prob["bc"] = list(bc)
+    if filt:
+        prob["filterfunction"] = P.exp_filter(4, 5.0)
+        prob["f_position"] = "substep"
+    return prob, sa, ss
+
+
+def forward_J(prob, mask, precision="fp64", state=None):
+    s = Adjoint(I.starmap_init(prob), precision=precision)
+    try:
+        dt = s.prepare()
+        if state:
+            for j, arr in state.items():
+                s.set_state(j, arr)
+        from starmap_amd.batch import count_steps
+        s.step(dt, count_steps(dt, s.par["tfinal"]))
+        return objective_value(s, mask)
+    finally:
+        s.close()
+
+
+def adjoint_of(prob, precision="fp64", every=4):
+    s = Adjoint(I.starmap_init(prob), precision=precision)
+    mask = strip_objective(s, xmin=0.2)
+    s.run({0: mask}, every=every)
+    return s, mask
+
+
+@pytest.mark.parametrize("bc", [(1, 1), (0, 0)])
+def test_lambda0_equals_finite_difference_in_initial_state(bc):
+    prob, _, _ = small(bc=bc, filt=True)
+    s, mask = adjoint_of(prob)
+    try:
+        rng = np.random.default_rng(1)
+        for j in (0, 1, 2, 5):
+            lam = s.lam(j)
+            shp = lam.shape
+            pts = [(0, 0, 0), (0, shp[1] - 1, shp[2] - 1), (0, 0, shp[2] - 1), (0, shp[1] - 1, 0)]
+            pts += [(0, int(rng.integers(shp[1])), int(rng.integers(shp[2]))) for _ in range(3)]
+            for p in pts:
+                e = np.zeros(shp)
+                e[p] = 1.0
+                Jp = forward_J(prob, mask, state={j: e})
+                Jm = forward_J(prob, mask, state={j: -e})
+                fd = (Jp - Jm) / 2
+                print("lambda", bc, j, p, lam[p], fd, flush=True)
+                assert abs(lam[p] - fd) <= 1e-9 * max(abs(fd), np.abs(lam).max()), (j, p, lam[p], fd)
+    finally:
+        s.close()
+
+
+def fd_cells(prob, sa, ss, cells, mask, which, rel=1e-6, oracle=False):
+    out = []
+    for c in cells:
+        vals = []
+        for sgn in (1, -1):
+            a, b = sa.copy(), ss.copy()
+            arr = a if which == "a" else b
+            eps = rel * arr.flat[c]
+            arr.flat[c] += sgn * eps
+            p2 = P.adjoint_grf(n=sa.shape[0], n_mom=prob["n_mom"], steps=None or 0, sigma_a=a,
+                               sigma_s0=b, src_std=0.2)
+            p2.update({k: prob[k] for k in ("bc", "tfinal") if k in prob})
+            for k in ("filterfunction", "f_position"):
+                if k in prob:
+                    p2[k] = prob[k]
+            if oracle:
+                ref, _ = so.starmap_solver(so.starmap_init(p2), verbose=False, return_state=True)
+                vals.append(float(np.sum(np.reshape(ref[0]["U"], mask.shape, order="F") * mask)))
+            else:
+                vals.append(forward_J(p2, mask))
+        out.append((vals[0] - vals[1]) / (2 * rel * (sa if which == "a" else ss).flat[c]))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("filt", [False, True])
+def test_material_gradients_match_finite_differences(filt):
+    prob, sa, ss = small(filt=filt)
+    s, mask = adjoint_of(prob, every=2)
+    try:
+        n = sa.shape[0]
+        owner = dict(prob, sigma_a=(lambda x, y: prob["_owner"](x, y).astype(np.float64)))
+        ga, gs = s.cell_gradients(owner, n * n)
+    finally:
+        s.close()
+    cells = [0, n - 1, n * n - 1, 5 * n + 7, 7 * n + 3, 9 * n + 10]
+    for which, g in (("a", ga), ("s", gs)):
+        fd = fd_cells(prob, sa, ss, cells, mask, which)
+        print("dJ/dsigma_" + which, g[cells], fd, flush=True)
+        scale = np.abs(fd).max()
+        assert np.all(np.abs(g[cells] - fd) <= 1e-5 * scale), (which, g[cells], fd)
+
+
+def test_material_gradient_matches_oracle_differences():
+    prob, sa, ss = small(n=10, steps=6)
+    s, mask = adjoint_of(prob, every=3)
+    try:
+        owner = dict(prob, sigma_a=(lambda x, y: prob["_owner"](x, y).astype(np.float64)))
+        ga, gs = s.cell_gradients(owner, 100)
+    finally:
+        s.close()
+    cells = [44, 57, 91]
+    for which, g in (("a", ga), ("s", gs)):
+        fd = fd_cells(prob, sa, ss, cells, mask, which, oracle=True)
+        print("oracle dJ/dsigma_" + which, g[cells], fd, flush=True)
+        assert np.all(np.abs(g[cells] - fd) <= 1e-5 * np.abs(fd).max())
+
+
+def test_fp32_adjoint_close_to_fp64():
+    prob, sa, ss = small(n=24, n_mom=5, steps=20)
+    res = {}
+    for prec in ("fp64", "fp32"):
+        s, mask = adjoint_of(prob, precision=prec, every=5)
+        try:
+            owner = dict(prob, sigma_a=(lambda x, y: prob["_owner"](x, y).astype(np.float64)))
+            res[prec] = s.cell_gradients(owner, 24 * 24)
+        finally:
+            s.close()
+    for a64, a32 in zip(res["fp64"], res["fp32"]):
+        assert np.linalg.norm(a32 - a64) <= 1e-4 * np.linalg.norm(a64)
+
+
+def test_torch_autograd_function():
+    torch = pytest.importorskip("torch")
+    from starmap_amd.adjoint import StarmapObjective
+    n = 12
+    prob0, sa, ss = small(n=n)
+
+    def make_prob(a, b):
+        p = P.adjoint_grf(n=n, n_mom=3, steps=9, sigma_a=a, sigma_s0=b, src_std=0.2)
+        return p
+
+    ta = torch.tensor(sa, requires_grad=True)
+    ts = torch.tensor(ss, requires_grad=True)
+    J = StarmapObjective.apply(ta, ts, make_prob, "fp64", 4)
+    J.backward()
+    s = Adjoint(I.starmap_init(make_prob(sa, ss)), precision="fp64")
+    try:
+        mask = strip_objective(s, xmin=0.8)
+        s.run({0: mask}, every=4)
+        assert abs(objective_value(s, mask) - float(J)) <= 1e-12 * abs(float(J))
+        owner = dict(prob0, sigma_a=(lambda x, y: prob0["_owner"](x, y).astype(np.float64)))
+        ga, gs = s.cell_gradients(owner, n * n)
+    finally:
+        s.close()
+    assert np.allclose(ta.grad.numpy().ravel(), ga, rtol=1e-12, atol=0)
+    assert np.allclose(ts.grad.numpy().ravel(), gs, rtol=1e-12, atol=0)
