@@ -17,7 +17,9 @@ cores, a bounded sample of the same workload).  A second leg ("gauss3d")
 measures the 3D P_7 (64 moments) 200^3 periodic Gaussian problem in FP64 and
 FP32 state; a third ("batched") 64 seeded 2D problems of uneven size, order
 and filter strength in FP32 state in the persistent batched kernel, against
-the FP64 plain rate and each problem's FP64 solve.
+the FP64 plain rate and each problem's FP64 solve; a fourth ("adjoint") the
+cost of the discrete adjoint of 2000 steps of a 2D P_15 512^2 problem and
+its gradients against FP64 finite differences.
 """
 import argparse
 import json
@@ -193,6 +195,81 @@ def batched_leg(fp64_plain_value, n_instances=64, check=True):
     return leg
 
 
+def adjoint_leg(n=512, n_mom=15, steps=2000, every=50, fd_cells=8, seed=2027):
+    """2D P_15 (136 moments) on 512^2, extrapolation BC, per-cell
+    sigma_a = 0.5*exp(Gaussian random field, correlation length 0.2, std
+    0.5), sigma_s0 = 1, Gaussian source (std 0.05) at (-0.5,0), 2000 steps,
+    FP32 state: the forward alone, then the discrete adjoint (checkpoint
+    every 50 steps, FP64 gradient accumulation) of J = h^2 * sum of U1(T)
+    over x > 0.8; per-cell dJ/dsigma_a, dJ/dsigma_s0 against FP64 central
+    differences (relative step 1e-6) at seeded cells."""
+    from starmap_amd import init
+    from starmap_amd import problems as P
+    from starmap_amd.adjoint import Adjoint, objective_value, strip_objective
+    prob = P.adjoint_grf(n=n, n_mom=n_mom, steps=steps)
+    par = init.starmap_init(prob)
+    log("adjoint: %dx%d P%d, %d steps" % (n, n, n_mom, steps))
+    s = Adjoint(par, precision="fp32")
+    try:
+        mask = strip_objective(s, xmin=0.8)
+        dt = s.prepare()
+        s.step(dt, steps)
+        fwd_ms = s.timing()[0]
+        J32 = objective_value(s, mask)
+        s.run({0: mask}, n_steps=steps, every=every)
+        f_ms, r_ms = s.adjoint_timing()
+        owner = dict(prob, sigma_a=(lambda x, y: prob["_owner"](x, y).astype(np.float64)))
+        ga, gs = s.cell_gradients(owner, n * n)
+    finally:
+        s.close()
+    log("adjoint: forward %.2f s, adjoint forward+reverse %.2f s" % (fwd_ms / 1e3, (f_ms + r_ms) / 1e3))
+    leg = {
+        "workload": "2D P%d %dx%d, %d steps, FP32 state, FP64 gradients" % (n_mom, n, n, steps),
+        "forward_ms": fwd_ms,
+        "adjoint_forward_ms": f_ms,
+        "adjoint_reverse_ms": r_ms,
+        "cost_vs_forward": (f_ms + r_ms) / fwd_ms,
+        "checkpoint_every": every,
+        "J": J32,
+    }
+    if fd_cells > 0:
+        rng = np.random.default_rng(seed)
+        cells = rng.choice(n * n, size=fd_cells, replace=False)
+        sa0 = np.asarray(0.5 * np.exp(P.gaussian_random_field(n)))
+        ss0 = np.ones((n, n))
+        fd = {"a": [], "s": []}
+        for which in ("a", "s"):
+            for c in cells:
+                vals = []
+                base = sa0 if which == "a" else ss0
+                eps = 1e-6 * base.flat[c]
+                for sgn in (1, -1):
+                    a, b = sa0.copy(), ss0.copy()
+                    (a if which == "a" else b).flat[c] += sgn * eps
+                    h64 = Adjoint(init.starmap_init(P.adjoint_grf(n=n, n_mom=n_mom, steps=steps,
+                                                                  sigma_a=a, sigma_s0=b)),
+                                  precision="fp64")
+                    try:
+                        d = h64.prepare()
+                        h64.step(d, steps)
+                        vals.append(objective_value(h64, mask))
+                    finally:
+                        h64.close()
+                fd[which].append((vals[0] - vals[1]) / (2 * eps))
+            log("adjoint: finite differences of sigma_%s done" % which)
+        res = {}
+        for which, g in (("a", ga), ("s", gs)):
+            f = np.array(fd[which])
+            ad = g[cells]
+            scale = np.abs(f).max()
+            res["sigma_" + which] = {
+                "max_abs_err_over_max_fd": float(np.abs(ad - f).max() / scale) if scale else None,
+                "adjoint": ad.tolist(), "finite_difference": f.tolist()}
+        leg["fd_check"] = dict(res, cells=[int(c) for c in cells], rel_step=1e-6,
+                               forward_precision="fp64")
+    return leg
+
+
 def cpu_baseline(prob, steps):
     """The FP64 oracle (C restatement of the reference step, OpenMP over
     components) timed on the host: `steps` full steps of the same workload."""
@@ -225,6 +302,10 @@ def main():
                     help="problems of the batched leg (0 skips it)")
     ap.add_argument("--no-batch-check", action="store_true",
                     help="skip the FP64 solve of every batched problem")
+    ap.add_argument("--adjoint-steps", type=int, default=2000,
+                    help="steps of the adjoint leg (0 skips it)")
+    ap.add_argument("--fd-cells", type=int, default=8,
+                    help="cells of the adjoint leg's finite-difference check")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the time-stepping core is a single-GPU "
@@ -324,6 +405,8 @@ def main():
     if args.batch > 0:
         out["batched"] = batched_leg(out["value"] if main_key == "fp64" else None,
                                      n_instances=args.batch, check=not args.no_batch_check)
+    if args.adjoint_steps > 0:
+        out["adjoint"] = adjoint_leg(steps=args.adjoint_steps, fd_cells=args.fd_cells)
 
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))

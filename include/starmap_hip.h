@@ -224,15 +224,16 @@ int smp_adjoint_seed(smp_problem *h, int32_t j, const double *src, int64_t count
  * runs the steps forward -- the state ends at the final time, as after
  * smp_step -- keeping a copy of the state every checkpoint_every steps, then
  * sweeps back from the seeded lambda(T), recomputing each segment from its
- * checkpoint.  Afterwards lambda holds dJ/dU(0) and the accumulators hold
- * dJ/dsigma_j for every component j at every point of its grid (FP64),
- * where sigma_j is sigma_a for j = 1 and sigma_t of j's grid type and order
- * otherwise.  Filters must be off or at 'substep' position and constant in
- * space; sources must be fields. */
+ * checkpoint.  Afterwards lambda holds dJ/dU(0) and, accumulated in FP64
+ * over the steps, dJ/dsigma_a and dJ/dsigma_s0 at every point of every
+ * staggered grid type (sigma_t = sigma_a + sigma_s0 - sigma_sm for j > 1,
+ * sigma_a for j = 1, starmap_solver.m:214-252).  Filters must be off or at
+ * 'substep' position and constant in space; sources must be fields. */
 int smp_adjoint(smp_problem *h, double dt, int64_t n_steps, int32_t checkpoint_every);
 
-/* Component j of the adjoint state (what = 0) or of dJ/dsigma_j (what = 1),
- * reference layout. */
+/* Component j of the adjoint state (what = 0), or dJ/dsigma_a (what = 1) or
+ * dJ/dsigma_s0 (what = 2) on the grid type of component j; reference
+ * layout. */
 int smp_adjoint_get(smp_problem *h, int32_t what, int32_t j, double *dst, int64_t count);
 
 /* Milliseconds of the last smp_adjoint: forward pass and reverse sweep

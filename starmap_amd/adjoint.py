@@ -3,12 +3,11 @@ materials, by the discrete adjoint of the time stepper (``smp_adjoint``).
 
 ``Adjoint(par)`` wraps a device handle.  ``run(seed, n_steps, every)`` runs
 ``n_steps`` steps forward from the initial state (keeping a state copy every
-``every`` steps) and back, and returns dJ/dsigma_j for every component;
-``material_gradients`` sums them to dJ/dsigma_a and dJ/dsigma_s0 on each
-staggered grid type (sigma_t = sigma_a + sigma_s0 - sigma_sm(m) for j > 1,
-sigma_a for j = 1, starmap_solver.m:214-252), and ``cell_gradients`` to
-per-cell values through the map from grid points to owning cells that the
-material functions use.
+``every`` steps) and back; ``material_gradients`` returns dJ/dsigma_a and
+dJ/dsigma_s0 on each staggered grid type (sigma_t = sigma_a + sigma_s0 -
+sigma_sm(m) for j > 1, sigma_a for j = 1, starmap_solver.m:214-252), and
+``cell_gradients`` sums them to per-cell values through the map from grid
+points to owning cells that the material functions use.
 
 ``StarmapObjective`` is a ``torch.autograd.Function`` over the same library:
 per-cell sigma_a and sigma_s0 in, the objective out, gradients from the
@@ -46,8 +45,8 @@ class Adjoint(HipSolver):
         return self._get(0, j)
 
     def dsigma(self, j):
-        """dJ/dsigma_j on the grid of component j."""
-        return self._get(1, j)
+        """(dJ/dsigma_a, dJ/dsigma_s0) on the grid type of component j."""
+        return self._get(1, j), self._get(2, j)
 
     def adjoint_timing(self):
         f, r = ctypes.c_double(), ctypes.c_double()
@@ -72,13 +71,8 @@ class Adjoint(HipSolver):
         """{grid type: (dJ/dsigma_a, dJ/dsigma_s0)} on each staggered grid."""
         out = {}
         for j in range(self.n_sys):
-            g = self.dsigma(j)
-            key = self.gkey[j]
-            ga, gs = out.get(key, (0.0, 0.0))
-            ga = ga + g
-            if j > 0:
-                gs = gs + g
-            out[key] = (ga, gs)
+            if self.gkey[j] not in out:
+                out[self.gkey[j]] = self.dsigma(j)
         return out
 
     def cell_gradients(self, owner_par, n_cells):

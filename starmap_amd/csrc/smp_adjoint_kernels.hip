@@ -19,6 +19,12 @@
 
 namespace smp {
 
+#define SMP_GLOBAL __attribute__((address_space(1)))
+template <typename P>
+__device__ __forceinline__ SMP_GLOBAL P *glb(P *p) {
+    return (SMP_GLOBAL P *)p;
+}
+
 namespace {
 
 __device__ __forceinline__ int wrapc(int i, int n, int bc, int ext) {
@@ -44,65 +50,79 @@ __device__ __forceinline__ double expm1div_d(double x, double *dedx) {
 
 template <typename T>
 __global__ __launch_bounds__(256) void adj_update_kernel(const AdjArgs<T> a) {
-    const int j = a.list[blockIdx.y];
-    const AdjComp<T> c = a.comp[j];
+    const int grp = blockIdx.y;
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int i0 = (int)(t % a.E[0]);
     const int64_t r = t / a.E[0];
     const int i1 = (int)(r % a.E[1]);
     const int i2 = (int)(r / a.E[1]);
-    if (i2 >= a.E[2] || i0 >= c.ext[0] || i1 >= c.ext[1] || i2 >= c.ext[2]) return;
+    const int q0 = a.grp0[grp], q1 = a.grp0[grp + 1];
+    const SMP_GLOBAL AdjComp<T> *comp = glb(a.comp);
+    const SMP_GLOBAL int32_t *list = glb(a.list);
+    {
+        const int jf = list[q0];
+        if (i2 >= a.E[2] || i0 >= comp[jf].ext[0] || i1 >= comp[jf].ext[1] || i2 >= comp[jf].ext[2])
+            return;
+    }
     const int ii[3] = {i0, i1, i2};
     const int64_t p = i0 + (int64_t)a.pitch0 * (i1 + (int64_t)a.ext1 * i2);
-    // W of the forward step (x, y, z terms in the forward order)
-    double w = 0;
-    for (int q = c.r0; q < c.r1; ++q) {
-        const AdjTerm tm = a.rows[q];
-        const AdjComp<T> &s = a.comp[tm.k];
-        int hi[3] = {ii[0], ii[1], ii[2]}, lo[3] = {ii[0], ii[1], ii[2]};
-        const int e = tm.axis;
-        if (s.cell[e]) {
-            lo[e] = ii[e] - 1;   // face target between cells i-1 and i
-        } else {
-            hi[e] = ii[e] + 1;   // cell target between faces i and i+1
-        }
-        const int ext = s.ext[e];
-        hi[e] = wrapc(hi[e], a.n[e], a.bc[e], ext);
-        lo[e] = wrapc(lo[e], a.n[e], a.bc[e], ext);
-        const T *Uk = a.U + (int64_t)tm.k * a.slab;
-        const double vh = (double)Uk[hi[0] + (int64_t)a.pitch0 * (hi[1] + (int64_t)a.ext1 * hi[2])];
-        const double vl = (double)Uk[lo[0] + (int64_t)a.pitch0 * (lo[1] + (int64_t)a.ext1 * lo[2])];
-        w += tm.coef * (vh - vl);
-    }
     const double hd = a.half_dt;
-    const double sg = c.sig ? (double)c.sig[p] : c.sig_s;
-    const double E = c.E ? (double)c.E[p] : c.E_s;
-    double dedx;
-    (void)expm1div_d(-sg * hd, &dedx);
-    const double dE = -hd * dedx;
-    const double wq = w + (c.Q ? (double)c.Q[p] : 0.0);
-    const int64_t jp = (int64_t)j * a.slab + p;
-    // states before each application (the forward rounds to T after each)
-    double u[2];
-    u[0] = (double)a.U0[jp];
-    u[1] = (double)(T)((u[0] + (hd * (wq - sg * u[0])) * E) * c.F[a.fslot[0] ? 1 : 0]);
-    double lam = (double)a.lam[jp];
-    double mu = 0, g = 0;
-    for (int app = a.napp - 1; app >= 0; --app) {
-        const double l = c.F[a.fslot[app] ? 1 : 0] * lam;
-        mu += hd * E * l;
-        g += l * (-hd * u[app] * E + hd * (wq - sg * u[app]) * dE);
-        lam = (1 - hd * sg * E) * l;
+    const SMP_GLOBAL AdjTerm *rows = glb(a.rows);
+    double ga = 0, gs = 0;
+    for (int q = q0; q < q1; ++q) {
+        const int j = list[q];
+        const SMP_GLOBAL AdjComp<T> &c = comp[j];
+        // W of the forward step (x, y, z terms in the forward order)
+        double w = 0;
+        for (int m = c.r0; m < c.r1; ++m) {
+            const SMP_GLOBAL AdjTerm &tm = rows[m];
+            const int e = tm.axis;
+            int hi[3] = {ii[0], ii[1], ii[2]}, lo[3] = {ii[0], ii[1], ii[2]};
+            if (comp[tm.k].cell[e])
+                lo[e] = ii[e] - 1;   // face target between cells i-1 and i
+            else
+                hi[e] = ii[e] + 1;   // cell target between faces i and i+1
+            const int ext = comp[tm.k].ext[e];
+            hi[e] = wrapc(hi[e], a.n[e], a.bc[e], ext);
+            lo[e] = wrapc(lo[e], a.n[e], a.bc[e], ext);
+            const SMP_GLOBAL T *Uk = glb(a.U) + (int64_t)tm.k * a.slab;
+            const double vh = (double)Uk[hi[0] + (int64_t)a.pitch0 * (hi[1] + (int64_t)a.ext1 * hi[2])];
+            const double vl = (double)Uk[lo[0] + (int64_t)a.pitch0 * (lo[1] + (int64_t)a.ext1 * lo[2])];
+            w += tm.coef * (vh - vl);
+        }
+        const double sg = c.sig ? (double)glb(c.sig)[p] : c.sig_s;
+        const double E = c.E ? (double)glb(c.E)[p] : c.E_s;
+        double dedx;
+        (void)expm1div_d(-sg * hd, &dedx);
+        const double dE = -hd * dedx;
+        const double wq = w + (c.Q ? (double)glb(c.Q)[p] : 0.0);
+        const int64_t jp = (int64_t)j * a.slab + p;
+        // states before each application (the forward rounds to T after each)
+        double u[2];
+        u[0] = (double)glb(a.U0)[jp];
+        u[1] = (double)(T)((u[0] + (hd * (wq - sg * u[0])) * E) * c.F[a.fslot[0] ? 1 : 0]);
+        double lam = (double)glb(a.lam)[jp];
+        double mu = 0, g = 0;
+        for (int app = a.napp - 1; app >= 0; --app) {
+            const double l = c.F[a.fslot[app] ? 1 : 0] * lam;
+            mu += hd * E * l;
+            g += l * (-hd * u[app] * E + hd * (wq - sg * u[app]) * dE);
+            lam = (1 - hd * sg * E) * l;
+        }
+        glb(a.lam)[jp] = (T)lam;
+        glb(a.mu)[jp] = (T)mu;
+        ga += g;
+        if (j > 0) gs += g;
     }
-    a.lam[jp] = (T)lam;
-    a.mu[jp] = (T)mu;
-    a.G[jp] += g;
+    const int64_t gp = (int64_t)a.gcode[grp] * a.slab + p;
+    glb(a.Ga)[gp] += ga;
+    glb(a.Gs)[gp] += gs;
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void adj_transpose_kernel(const AdjArgs<T> a) {
-    const int k = a.list[blockIdx.y];
-    const AdjComp<T> c = a.comp[k];
+    const int k = glb(a.list)[blockIdx.y];
+    const SMP_GLOBAL AdjComp<T> &c = glb(a.comp)[k];
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int x0 = (int)(t % a.E[0]);
     const int64_t r = t / a.E[0];
@@ -112,10 +132,10 @@ __global__ __launch_bounds__(256) void adj_transpose_kernel(const AdjArgs<T> a) 
     const int xx[3] = {x0, x1, x2};
     double acc = 0;
     for (int q = c.c0; q < c.c1; ++q) {
-        const AdjTerm tm = a.cols[q];
+        const SMP_GLOBAL AdjTerm &tm = glb(a.cols)[q];
         const int e = tm.axis;
         const int n = a.n[e];
-        const T *mj = a.mu + (int64_t)tm.k * a.slab;
+        const SMP_GLOBAL T *mj = glb(a.mu) + (int64_t)tm.k * a.slab;
         // target indices whose high / low sample is x, and whether they count
         int ih[3] = {xx[0], xx[1], xx[2]}, il[3] = {xx[0], xx[1], xx[2]};
         bool use_h, use_l;
@@ -149,7 +169,7 @@ __global__ __launch_bounds__(256) void adj_transpose_kernel(const AdjArgs<T> a) 
         acc += tm.coef * d;
     }
     const int64_t kp = (int64_t)k * a.slab + x0 + (int64_t)a.pitch0 * (x1 + (int64_t)a.ext1 * x2);
-    a.lam[kp] = (T)((double)a.lam[kp] + acc);
+    glb(a.lam)[kp] = (T)((double)glb(a.lam)[kp] + acc);
 }
 
 template <typename T>
@@ -160,23 +180,23 @@ __global__ void copy_kernel(T *dst, const T *src, int64_t count) {
 }
 
 template <typename T>
-static dim3 adj_grid(const AdjArgs<T> &a) {
+static dim3 adj_grid(const AdjArgs<T> &a, int y) {
     const int64_t cells = (int64_t)a.E[0] * a.E[1] * a.E[2];
-    return dim3((unsigned)((cells + 255) / 256), (unsigned)a.n_list);
+    return dim3((unsigned)((cells + 255) / 256), (unsigned)y);
 }
 
 template <typename T>
 hipError_t launch_adj_update(const AdjArgs<T> &a, hipStream_t s) {
     if (a.n_list <= 0) return hipSuccess;
     if (a.napp != 1 && a.napp != 2) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(adj_update_kernel<T>, adj_grid(a), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(adj_update_kernel<T>, adj_grid(a, a.n_grp), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_adj_transpose(const AdjArgs<T> &a, hipStream_t s) {
     if (a.n_list <= 0) return hipSuccess;
-    hipLaunchKernelGGL(adj_transpose_kernel<T>, adj_grid(a), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(adj_transpose_kernel<T>, adj_grid(a, a.n_list), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

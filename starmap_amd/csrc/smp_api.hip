@@ -107,6 +107,9 @@ struct smp_problem {
     // (FP64), tables of the adjoint kernels, segment checkpoints and the
     // states recorded while a segment is recomputed
     void *lam = nullptr, *mu = nullptr, *G = nullptr, *work = nullptr;
+    int adj_ngrp[2] = {0, 0};     // odd, even: grid types, list ranges, codes
+    int32_t adj_grp0[2][9] = {};
+    int32_t adj_gcode[2][8] = {};
     void *adj_comp = nullptr, *adj_rows = nullptr, *adj_cols = nullptr, *adj_list = nullptr;
     std::vector<void *> adj_seg, adj_rec;
     double adj_fwd_ms = 0, adj_rev_ms = 0;
@@ -938,8 +941,27 @@ int adjoint_tables(smp_problem *h) {
         cols.insert(cols.end(), by_src[k].begin(), by_src[k].end());
         comp[k].c1 = (int32_t)cols.size();
     }
-    std::vector<int32_t> list(g.odd.begin(), g.odd.end());
-    list.insert(list.end(), g.even.begin(), g.even.end());
+    // components of each parity grouped by grid type: odd first, then even
+    std::vector<int32_t> list;
+    for (int par = 0; par < 2; ++par) {
+        const std::vector<int32_t> &cs = par == 0 ? g.odd : g.even;
+        std::vector<int> codes;
+        for (int j : cs) {
+            const int code = (g.gt[3 * j] - 1) * 4 + (g.gt[3 * j + 1] - 1) * 2 + (g.gt[3 * j + 2] - 1);
+            if (std::find(codes.begin(), codes.end(), code) == codes.end()) codes.push_back(code);
+        }
+        h->adj_ngrp[par] = (int)codes.size();
+        const int base = (int)list.size();
+        for (size_t q = 0; q < codes.size(); ++q) {
+            h->adj_grp0[par][q] = (int32_t)list.size() - base;
+            h->adj_gcode[par][q] = codes[q];
+            for (int j : cs) {
+                const int code = (g.gt[3 * j] - 1) * 4 + (g.gt[3 * j + 1] - 1) * 2 + (g.gt[3 * j + 2] - 1);
+                if (code == codes[q]) list.push_back(j);
+            }
+        }
+        h->adj_grp0[par][codes.size()] = (int32_t)list.size() - base;
+    }
     for (void **p : {&h->adj_comp, &h->adj_rows, &h->adj_cols, &h->adj_list}) {
         h->dfree(*p);
         *p = nullptr;
@@ -970,9 +992,10 @@ int adjoint_buffers(smp_problem *h) {
         }
     }
     if (!h->G) {
-        int rc = h->dalloc(&h->G, (size_t)h->slab * 8 * h->geo.n_sys);
+        // dJ/dsigma_a and dJ/dsigma_s0 for each of the 8 grid type codes
+        int rc = h->dalloc(&h->G, (size_t)h->slab * 8 * 16);
         if (rc) return rc;
-        HIP_TRY(hipMemsetAsync(h->G, 0, (size_t)h->slab * 8 * h->geo.n_sys, h->stream));
+        HIP_TRY(hipMemsetAsync(h->G, 0, (size_t)h->slab * 8 * 16, h->stream));
     }
     HIP_TRY(hipStreamSynchronize(h->stream));
     return SMP_OK;
@@ -1029,6 +1052,7 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
     const int32_t *list_even = list_odd + nl_odd;
     auto adj_args = [&](const void *U, const void *U0, const int32_t *list, int nl, int napp, int f0,
                         int f1) {
+        const int par = list == list_odd ? 0 : 1;
         smp::AdjArgs<T> a;
         std::memset(&a, 0, sizeof(a));
         a.comp = (const smp::AdjComp<T> *)h->adj_comp;
@@ -1036,11 +1060,15 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
         a.cols = (const smp::AdjTerm *)h->adj_cols;
         a.list = list;
         a.n_list = nl;
+        a.n_grp = h->adj_ngrp[par];
+        for (int q = 0; q < 9; ++q) a.grp0[q] = h->adj_grp0[par][q];
+        for (int q = 0; q < 8; ++q) a.gcode[q] = h->adj_gcode[par][q];
         a.U = (const T *)U;
         a.U0 = (const T *)U0;
         a.lam = (T *)h->lam;
         a.mu = (T *)h->mu;
-        a.G = (double *)h->G;
+        a.Ga = (double *)h->G;
+        a.Gs = (double *)h->G + 8 * h->slab;
         a.slab = h->slab;
         a.pitch0 = h->pitch0;
         a.ext1 = h->E[1];
@@ -1636,7 +1664,7 @@ int smp_adjoint_reset(smp_problem *h) {
     int rc = adjoint_buffers(h);
     if (rc) return rc;
     HIP_TRY(hipMemsetAsync(h->lam, 0, (size_t)h->slab * h->elem * h->geo.n_sys, h->stream));
-    HIP_TRY(hipMemsetAsync(h->G, 0, (size_t)h->slab * 8 * h->geo.n_sys, h->stream));
+    HIP_TRY(hipMemsetAsync(h->G, 0, (size_t)h->slab * 8 * 16, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     return SMP_OK;
 }
@@ -1679,16 +1707,18 @@ int smp_adjoint(smp_problem *h, double dt, int64_t n_steps, int32_t checkpoint_e
 int smp_adjoint_get(smp_problem *h, int32_t what, int32_t j, double *dst, int64_t count) {
     if (!h || !dst) return fail(SMP_E_INVALID, "null argument");
     REQUIRE_DEVICE(h);
-    if (what != 0 && what != 1) return fail(SMP_E_INVALID, "what must be 0 (adjoint state) or 1 (gradient)");
+    if (what < 0 || what > 2)
+        return fail(SMP_E_INVALID, "what must be 0 (adjoint state), 1 (dJ/dsigma_a) or 2 (dJ/dsigma_s0)");
     if (j < 0 || j >= h->geo.n_sys) return fail(SMP_E_INVALID, "component out of range");
     if (count != h->numel(j)) return fail(SMP_E_INVALID, "count does not match the component shape");
     if (!h->lam) return fail(SMP_E_STATE, "no adjoint was run or seeded");
     HIP_TRY(hipSetDevice(h->device));
     HIP_TRY(hipStreamSynchronize(h->stream));
     const int gt[3] = {h->geo.gt[3 * j], h->geo.gt[3 * j + 1], h->geo.gt[3 * j + 2]};
-    if (what == 1) {
+    if (what >= 1) {
+        const int code = (gt[0] - 1) * 4 + (gt[1] - 1) * 2 + (gt[2] - 1) + (what == 2 ? 8 : 0);
         std::vector<double> tmp(h->slab);
-        HIP_TRY(hipMemcpy(tmp.data(), (const char *)h->G + (size_t)j * h->slab * 8,
+        HIP_TRY(hipMemcpy(tmp.data(), (const char *)h->G + (size_t)code * h->slab * 8,
                           (size_t)h->slab * 8, hipMemcpyDeviceToHost));
         h->unpack<double>(gt, tmp.data(), dst);
     } else if (h->precision == SMP_FP64) {

@@ -145,13 +145,17 @@ struct AdjArgs {
     const AdjComp<T> *comp;
     const AdjTerm *rows;      // terms by target: k = source component
     const AdjTerm *cols;      // terms by source: k = target component
-    const int32_t *list;      // components of the parity this launch updates
-    int32_t n_list;
+    const int32_t *list;      // components of the parity this launch updates,
+    int32_t n_list;           // grouped by grid type
+    int32_t n_grp;            // grid types in the list
+    int32_t grp0[9];          // list range of each grid type
+    int32_t gcode[8];         // grid type code (gtx-1)*4+(gty-1)*2+(gtz-1)
     const T *U;               // forward state during the substep (sources)
     const T *U0;              // targets before the substep (rows kernel)
     T *lam;                   // adjoint state
     T *mu;                    // adjoint of the flux W of the targets
-    double *G;                // dJ/dsigma per component and point (FP64)
+    double *Ga, *Gs;          // dJ/dsigma_a, dJ/dsigma_s0 per grid type code
+                              // and point (8 slabs each, FP64)
     int64_t slab;
     int32_t pitch0, ext1;
     int32_t E[3];             // storage extents per kernel axis
@@ -162,7 +166,9 @@ struct AdjArgs {
 };
 
 // adjoint of the half-step applications of one substep for the targets in
-// a.list: mu = B*lambda', lambda = A*lambda', G += lambda'*(dA u + dB (W+Q))
+// a.list: mu = B*lambda', lambda = A*lambda', and per grid type the sum over
+// its targets of lambda'*(dA u + dB (W+Q)) added to Ga (all targets) and Gs
+// (targets j > 1): one thread per point and grid type
 template <typename T>
 hipError_t launch_adj_update(const AdjArgs<T> &a, hipStream_t s);
 
