@@ -151,8 +151,9 @@ def batched_leg(fp64_plain_value, n_instances=64, check=True):
     b = BatchSolver(pars, precision="fp32")
     try:
         dts, ns = b.prepare()
-        tile, bpc, times = autotune.tune(b, steps=2, use_cache=False)
-        log("batched: autotuner chose tile %s, %d workgroups/CU budget" % (tile, bpc))
+        (tile, bpc, cw), times = autotune.tune(b, steps=2, use_cache=False)
+        log("batched: autotuner chose tile %s, %d workgroups/CU budget, %d cell waves"
+            % (tile, bpc, cw))
         b.prepare()
         b.step(ns, use_graph=True)
         tot_ms, ker_ms, launches, by, mcu, grid = b.timing()
@@ -169,8 +170,8 @@ def batched_leg(fp64_plain_value, n_instances=64, check=True):
             "persistent_grid": grid,
             "achieved_GBs": by / (tot_ms / 1e3) / 1e9,
             "frac": by / (tot_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
-            "schedule": {"tile": list(tile), "blocks_per_cu": bpc,
-                         "candidates_ms": {"%s/%d" % (t, k): v for (t, k), v in times.items()}},
+            "schedule": {"tile": list(tile), "blocks_per_cu": bpc, "cell_waves": cw,
+                         "candidates_ms": {"%s/%d/%d" % c: v for c, v in times.items()}},
         }
         if check:
             worst, worst_all, worst_i = 0.0, 0.0, -1

@@ -83,6 +83,10 @@ typedef struct smp_desc {
     int32_t blocks_per_cu;    /* LDS budget: workgroups per CU the schedule is
                                  made for (1..8); 0 takes SMP_BLOCKS_PER_CU
                                  from the environment, else 1                 */
+    int32_t cell_waves;       /* waves updating the cells of a tile: 4, or 8
+                                 (two per SIMD, each with half of a group's
+                                 targets); 0 takes SMP_CELL_WAVES from the
+                                 environment, else 8 for FP32 and 4 for FP64 */
 } smp_desc;
 
 /* Last error message of the calling thread ("" if none). */
@@ -167,12 +171,14 @@ int smp_plan_stats(const smp_problem *h, int32_t kind, int32_t *n_groups, int32_
 /* ---- schedule (no counterpart in the reference) ------------------------ */
 
 /* Re-make the substep schedules of the handle for another tile shape (cells
- * per workgroup per kernel axis; NULL or zeros: library default) and LDS
- * budget (workgroups per CU, 1..8; 0 keeps the current one).  The state and
- * coefficients are kept; on failure the previous schedule stays in force.
- * Used by the autotuner. */
-int smp_set_schedule(smp_problem *h, const int32_t *tile, int32_t blocks_per_cu);
-int smp_get_schedule(const smp_problem *h, int32_t *tile, int32_t *blocks_per_cu);
+ * per workgroup per kernel axis; NULL or zeros: library default), LDS budget
+ * (workgroups per CU, 1..8; 0 keeps the current one) and cell-wave count (4
+ * or 8; 0 keeps the current one).  The state and coefficients are kept; on
+ * failure the previous schedule stays in force.  Used by the autotuner. */
+int smp_set_schedule(smp_problem *h, const int32_t *tile, int32_t blocks_per_cu,
+                     int32_t cell_waves);
+int smp_get_schedule(const smp_problem *h, int32_t *tile, int32_t *blocks_per_cu,
+                     int32_t *cell_waves);
 
 /* ---- state checkpoints in device memory (no counterpart in the reference) */
 

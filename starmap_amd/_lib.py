@@ -53,7 +53,8 @@ class Desc(ctypes.Structure):
                 ("precision", ctypes.c_int32),
                 ("device", ctypes.c_int32),
                 ("tile", ctypes.c_int32 * 3),
-                ("blocks_per_cu", ctypes.c_int32)]
+                ("blocks_per_cu", ctypes.c_int32),
+                ("cell_waves", ctypes.c_int32)]
 
 
 _lib = None
@@ -86,8 +87,8 @@ def lib():
         "smp_stream": (vp, [vp]),
         "smp_device_bytes": (i64, [vp]),
         "smp_plan_stats": (ctypes.c_int, [vp, i32, vp, vp, vp, vp]),
-        "smp_set_schedule": (ctypes.c_int, [vp, vp, i32]),
-        "smp_get_schedule": (ctypes.c_int, [vp, vp, vp]),
+        "smp_set_schedule": (ctypes.c_int, [vp, vp, i32, i32]),
+        "smp_get_schedule": (ctypes.c_int, [vp, vp, vp, vp]),
         "smp_checkpoint_save": (ctypes.c_int, [vp, i32]),
         "smp_checkpoint_load": (ctypes.c_int, [vp, i32]),
         "smp_checkpoint_free": (ctypes.c_int, [vp, i32]),

@@ -1,8 +1,9 @@
 """Autotuner for the substep schedules.
 
 For a prepared ``HipSolver`` or ``BatchSolver`` it tries candidate
-schedules -- the workgroup tile shape and the LDS budget (workgroups per
-CU the schedule is made for) -- by re-making the schedules of every handle
+schedules -- the workgroup tile shape, the LDS budget (workgroups per CU the
+schedule is made for) and the number of waves updating the cells (4, or 8
+with two per SIMD) -- by re-making the schedules of every handle
 (``smp_set_schedule``), timing a few steps with HIP events, and keeping the
 fastest.  The state is saved to a device checkpoint before the trials and
 restored after each one, so tuning does not advance the solution.  Results
@@ -20,6 +21,7 @@ TILES = {1: [(255, 1, 1)],
          2: [(63, 4, 1), (127, 2, 1), (31, 8, 1)],
          3: [(15, 4, 4), (31, 4, 2), (15, 8, 2)]}
 BLOCKS = (1, 2)
+CELL_WAVES = (4, 8)
 _SLOT = 4095            # checkpoint slot used while tuning
 
 
@@ -64,12 +66,12 @@ def _store_cache(key, entry):
 
 def candidates(target):
     dim = max(int(np.sum(np.asarray(s.n) > 1)) for s in _solvers(target))
-    return [(t, b) for t in TILES[max(dim, 1)] for b in BLOCKS]
+    return [(t, b, c) for t in TILES[max(dim, 1)] for b in BLOCKS for c in CELL_WAVES]
 
 
-def apply(target, tile, blocks_per_cu):
+def apply(target, tile, blocks_per_cu, cell_waves):
     for s in _solvers(target):
-        s.set_schedule(tile, blocks_per_cu)
+        s.set_schedule(tile, blocks_per_cu, cell_waves)
 
 
 def _time(target, steps):
@@ -83,24 +85,25 @@ def _time(target, steps):
 
 def tune(target, steps=2, use_cache=True, verbose=False):
     """Choose and apply the fastest schedule for a prepared target; returns
-    (tile, blocks_per_cu, {candidate: ms})."""
+    ((tile, blocks_per_cu, cell_waves), {candidate: ms})."""
     key = signature(target)
     if use_cache:
         hit = _load_cache().get(key)
         if hit:
-            apply(target, tuple(hit["tile"]), hit["blocks_per_cu"])
-            return tuple(hit["tile"]), hit["blocks_per_cu"], {}
+            best = (tuple(hit["tile"]), hit["blocks_per_cu"], hit["cell_waves"])
+            apply(target, *best)
+            return best, {}
     sol = _solvers(target)
     for s in sol:
         s.checkpoint_save(_SLOT)
     times = {}
     try:
-        for tile, b in candidates(target):
+        for cand in candidates(target):
             try:
-                apply(target, tile, b)
+                apply(target, *cand)
             except Exception as e:        # a schedule the coupling does not fit
                 if verbose:
-                    print("autotune: skip", tile, b, e)
+                    print("autotune: skip", cand, e)
                 continue
             _time(target, 1)             # first use: coefficient upload, warm-up
             for s in sol:
@@ -108,17 +111,17 @@ def tune(target, steps=2, use_cache=True, verbose=False):
             ms = _time(target, steps)
             for s in sol:
                 s.checkpoint_load(_SLOT)
-            times[(tile, b)] = ms
+            times[cand] = ms
             if verbose:
-                print("autotune:", tile, b, "%.3f ms" % ms)
+                print("autotune:", cand, "%.3f ms" % ms)
     finally:
         for s in sol:
             s.checkpoint_load(_SLOT)
             s.checkpoint_free(_SLOT)
     if not times:
         raise RuntimeError("autotune: no candidate schedule fits")
-    (tile, b), _ = min(times.items(), key=lambda kv: kv[1])
-    apply(target, tile, b)
-    _store_cache(key, {"tile": list(tile), "blocks_per_cu": b,
-                       "ms": {"%s/%d" % (t, bb): v for (t, bb), v in times.items()}})
-    return tile, b, times
+    best, _ = min(times.items(), key=lambda kv: kv[1])
+    apply(target, *best)
+    _store_cache(key, {"tile": list(best[0]), "blocks_per_cu": best[1], "cell_waves": best[2],
+                       "ms": {"%s/%d/%d" % c: v for c, v in times.items()}})
+    return best, times

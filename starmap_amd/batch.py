@@ -29,12 +29,12 @@ def count_steps(dt, tfinal):
 
 
 class BatchSolver:
-    def __init__(self, pars, precision="fp32", device=0, tile=None, blocks_per_cu=0):
+    def __init__(self, pars, precision="fp32", device=0, tile=None, blocks_per_cu=0, cell_waves=0):
         self.solvers = []
         try:
             for p in pars:
                 self.solvers.append(HipSolver(p, precision=precision, device=device, tile=tile,
-                                              blocks_per_cu=blocks_per_cu))
+                                              blocks_per_cu=blocks_per_cu, cell_waves=cell_waves))
         except Exception:
             self.close()
             raise

@@ -87,6 +87,7 @@ struct smp_problem {
     bool gen[2] = {false, false};  // some coefficient is read from global memory
     void *cf_dev[2] = {nullptr, nullptr};
     int blocks_per_cu = 0;
+    int cell_waves = 4;            // 4, or 8 (two waves per SIMD on every cell)
     bool host_only = false;
     std::vector<int32_t> mom_order, f_order;
     Field sA1, EA;
@@ -100,7 +101,7 @@ struct smp_problem {
     void *interp_buf = nullptr;
     int64_t interp_cap = 0;
     void *red_buf = nullptr;
-    void *sink = nullptr;         // 256 elements: stores of lanes without a target cell
+    void *sink = nullptr;         // 512 elements: stores of lanes without a target cell
     std::vector<hipEvent_t> events;
     std::vector<void *> ckpt;     // device copies of the whole state
     // discrete adjoint: lambda, mu (state layout), dJ/dsigma per component
@@ -373,6 +374,12 @@ int smp_create(const smp_desc *d, smp_problem **out) {
     int want_blocks = 1;
     if (const char *env = std::getenv("SMP_BLOCKS_PER_CU")) want_blocks = std::atoi(env);
     if (d->blocks_per_cu) want_blocks = d->blocks_per_cu;
+    // FP32 fits two cell waves per SIMD in registers; FP64 would spill
+    int cw = d->precision == SMP_FP32 ? 8 : 4;
+    if (const char *env = std::getenv("SMP_CELL_WAVES")) cw = std::atoi(env);
+    if (d->cell_waves) cw = d->cell_waves;
+    if (cw != 4 && cw != 8) return fail(SMP_E_INVALID, "cell_waves must be 4 or 8");
+    h->cell_waves = cw;
     if (want_blocks < 1 || want_blocks > 8) return fail(SMP_E_INVALID, "blocks_per_cu must be 1..8");
     h->blocks_per_cu = want_blocks;
     for (int kind = 0; kind < 2; ++kind) {
@@ -389,7 +396,7 @@ int smp_create(const smp_desc *d, smp_problem **out) {
     HIP_TRY(hipMemsetAsync(h->U, 0, (size_t)h->slab * h->elem * d->n_sys, h->stream));
     rc = h->dalloc(&h->red_buf, sizeof(double) * (3 * 1024 + 3));
     if (rc) return rc;
-    rc = h->dalloc(&h->sink, sizeof(double) * 256);
+    rc = h->dalloc(&h->sink, sizeof(double) * 512);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(h->stream));
     *out = h.release();
@@ -677,6 +684,11 @@ int refresh_coefs(smp_problem *h) {
             for (int q = 0; q < smp::MAXT; ++q) {
                 r.t[q].off = slots_base + r.t[q].off * h->elem;
                 r.t[q].stride *= h->elem;
+                if (h->elem == 4) {
+                    const float c = (float)r.t[q].coef;
+                    r.t[q].coef32[0] = c;
+                    r.t[q].coef32[1] = 0.0f;
+                }
             }
         }
         h->gen[kind] = false;
@@ -811,7 +823,7 @@ int run_steps(smp_problem *h, double dt, int64_t n_steps) {
         }
         const int kind = p == &even ? 1 : 0;
         if (li < timed) HIP_TRY(hipEventRecord(h->events[2 + 2 * li], h->stream));
-        HIP_TRY(smp::launch_substep<T>(*p, h->plan[kind].max_terms, h->gen[kind], smp::KTHREADS,
+        HIP_TRY(smp::launch_substep<T>(*p, h->plan[kind].max_terms, h->gen[kind], h->cell_waves,
                                        h->lds_bytes[kind], h->stream));
         if (li < timed) HIP_TRY(hipEventRecord(h->events[3 + 2 * li], h->stream));
         ++li;
@@ -1093,14 +1105,14 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
         const smp::SubstepArgs<T> s3 = make_args<T>(h, 0, dt, 1, 1, 1, h->work);
         for (int64_t s = 0; s < ns; ++s) {
             HIP_TRY(hipMemcpyAsync(h->adj_rec[2 * s], h->work, st, hipMemcpyDeviceToDevice, h->stream));
-            HIP_TRY(smp::launch_substep<T>(s1, h->plan[0].max_terms, h->gen[0], smp::KTHREADS,
+            HIP_TRY(smp::launch_substep<T>(s1, h->plan[0].max_terms, h->gen[0], h->cell_waves,
                                            h->lds_bytes[0], h->stream));
-            HIP_TRY(smp::launch_substep<T>(s2, h->plan[1].max_terms, h->gen[1], smp::KTHREADS,
+            HIP_TRY(smp::launch_substep<T>(s2, h->plan[1].max_terms, h->gen[1], h->cell_waves,
                                            h->lds_bytes[1], h->stream));
             HIP_TRY(hipMemcpyAsync(h->adj_rec[2 * s + 1], h->work, st, hipMemcpyDeviceToDevice,
                                    h->stream));
             if (s + 1 < ns)
-                HIP_TRY(smp::launch_substep<T>(s3, h->plan[0].max_terms, h->gen[0], smp::KTHREADS,
+                HIP_TRY(smp::launch_substep<T>(s3, h->plan[0].max_terms, h->gen[0], h->cell_waves,
                                                h->lds_bytes[0], h->stream));
         }
         for (int64_t s = ns - 1; s >= 0; --s) {
@@ -1150,7 +1162,7 @@ struct smp_batch {
     // captured launch sequence and the structure it was captured for
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    int g_launches = -1, g_grid = 0, g_nt = 0;
+    int g_launches = -1, g_grid = 0, g_nt = 0, g_cw = 0;
     size_t g_lds = 0;
     bool g_gen = false;
     std::vector<hipEvent_t> events;
@@ -1293,9 +1305,14 @@ int batch_run(smp_batch *b, const double *dt, const int64_t *ns, int use_graph) 
     HIP_TRY(hipMemcpyAsync(b->launch, desc.data(), sizeof(smp::BatchLaunch<T>) * n_launch,
                            hipMemcpyHostToDevice, b->stream));
     // persistent grid: every CU holds as many workgroups as fit
+    // one kernel variant for the whole batch: the problems' cell-wave count
+    const int cw = b->hs[0]->cell_waves;
+    for (int i = 1; i < n; ++i)
+        if (b->hs[i]->cell_waves != cw)
+            return fail(SMP_E_INVALID, "batched handles must share cell_waves");
     int ncu = 0, per_cu = 0;
     HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device));
-    HIP_TRY(smp::batch_occupancy<T>(nt, gen, lds, &per_cu));
+    HIP_TRY(smp::batch_occupancy<T>(nt, gen, cw, lds, &per_cu));
     if (per_cu < 1) return fail(SMP_E_INVALID, "the batched kernel does not fit on a CU");
     const int grid = ncu * per_cu;
     b->grid = grid;
@@ -1305,7 +1322,7 @@ int batch_run(smp_batch *b, const double *dt, const int64_t *ns, int use_graph) 
         HIP_TRY(smp::launch_fill(b->counters, n_launch, 0, b->stream));
         for (int L = 0; L < n_launch; ++L) {
             if (timed) HIP_TRY(hipEventRecord(b->events[2 + 2 * L], b->stream));
-            HIP_TRY(smp::launch_batch<T>(dl + L, nt, gen, grid, lds, b->stream));
+            HIP_TRY(smp::launch_batch<T>(dl + L, nt, gen, cw, grid, lds, b->stream));
             if (timed) HIP_TRY(hipEventRecord(b->events[3 + 2 * L], b->stream));
         }
         return SMP_OK;
@@ -1318,7 +1335,7 @@ int batch_run(smp_batch *b, const double *dt, const int64_t *ns, int use_graph) 
     }
     if (use_graph) {
         const bool same = b->gexec && b->g_launches == n_launch && b->g_grid == grid &&
-                          b->g_lds == lds && b->g_nt == nt && b->g_gen == gen;
+                          b->g_lds == lds && b->g_nt == nt && b->g_gen == gen && b->g_cw == cw;
         if (!same) {
             if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
             if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -1338,6 +1355,7 @@ int batch_run(smp_batch *b, const double *dt, const int64_t *ns, int use_graph) 
             b->g_lds = lds;
             b->g_nt = nt;
             b->g_gen = gen;
+            b->g_cw = cw;
         }
         HIP_TRY(hipEventRecord(b->events[0], b->stream));
         HIP_TRY(hipGraphLaunch(b->gexec, b->stream));
@@ -1519,9 +1537,12 @@ int smp_plan_stats(const smp_problem *h, int32_t kind, int32_t *n_groups, int32_
     return SMP_OK;
 }
 
-int smp_set_schedule(smp_problem *h, const int32_t *tile, int32_t blocks_per_cu) {
+int smp_set_schedule(smp_problem *h, const int32_t *tile, int32_t blocks_per_cu,
+                     int32_t cell_waves) {
     if (!h) return fail(SMP_E_INVALID, "null handle");
     if (blocks_per_cu < 0 || blocks_per_cu > 8) return fail(SMP_E_INVALID, "blocks_per_cu must be 0..8");
+    if (cell_waves != 0 && cell_waves != 4 && cell_waves != 8)
+        return fail(SMP_E_INVALID, "cell_waves must be 0, 4 or 8");
     if (!h->host_only) {
         HIP_TRY(hipSetDevice(h->device));
         HIP_TRY(hipStreamSynchronize(h->stream));
@@ -1557,15 +1578,18 @@ int smp_set_schedule(smp_problem *h, const int32_t *tile, int32_t blocks_per_cu)
         h->ntile[e] = (h->E[e] + h->ts.tile[e] - 1) / h->ts.tile[e];
         h->n_tiles *= h->ntile[e];
     }
+    if (cell_waves) h->cell_waves = cell_waves;
     h->coef_dirty = true;
     return SMP_OK;
 }
 
-int smp_get_schedule(const smp_problem *h, int32_t *tile, int32_t *blocks_per_cu) {
+int smp_get_schedule(const smp_problem *h, int32_t *tile, int32_t *blocks_per_cu,
+                     int32_t *cell_waves) {
     if (!h) return fail(SMP_E_INVALID, "null handle");
     if (tile)
         for (int e = 0; e < 3; ++e) tile[e] = h->ts.tile[e];
     if (blocks_per_cu) *blocks_per_cu = h->blocks_per_cu;
+    if (cell_waves) *cell_waves = h->cell_waves;
     return SMP_OK;
 }
 

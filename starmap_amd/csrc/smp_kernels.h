@@ -18,7 +18,8 @@ constexpr int BLOB_SZ = 3072;
 constexpr int BLOB_REC_OFF = 16;
 constexpr int BLOB_LD_OFF = BLOB_REC_OFF + GMAX * (int)sizeof(TgtRec);
 constexpr int CPS_MAX = 16;   // DMA chunks per LDS slot
-constexpr int KTHREADS = 384; // 4 cell waves + 2 copy waves
+// threads of a substep workgroup: cw cell waves (4 or 8) + 2 copy waves
+constexpr int kthreads(int cw) { return cw * 64 + 128; }
 
 template <typename T>
 struct SubstepArgs {
@@ -67,20 +68,21 @@ size_t substep_lds_overhead();
 
 // nt: terms per target the kernel unrolls (8 or 12); gen: some coefficient
 // is a field read from global memory (more fields than the LDS stages, or a
-// spatially varying filter)
+// spatially varying filter); cw: cell waves (4, or 8 with two waves per
+// SIMD on every cell)
 template <typename T>
-hipError_t launch_substep(const SubstepArgs<T> &a, int nt, bool gen, int threads,
-                          size_t lds_bytes, hipStream_t s);
+hipError_t launch_substep(const SubstepArgs<T> &a, int nt, bool gen, int cw, size_t lds_bytes,
+                          hipStream_t s);
 
 // Persistent batched substep launch over `grid` workgroups; bl is a device
 // pointer.  nt / gen as for launch_substep, over all entries.
 template <typename T>
-hipError_t launch_batch(const BatchLaunch<T> *bl, int nt, bool gen, int grid, size_t lds_bytes,
-                        hipStream_t s);
+hipError_t launch_batch(const BatchLaunch<T> *bl, int nt, bool gen, int cw, int grid,
+                        size_t lds_bytes, hipStream_t s);
 
 // workgroups of the batched kernel resident per CU with lds_bytes of LDS
 template <typename T>
-hipError_t batch_occupancy(int nt, bool gen, size_t lds_bytes, int *blocks);
+hipError_t batch_occupancy(int nt, bool gen, int cw, size_t lds_bytes, int *blocks);
 
 // p[0..n) = v (a kernel, so that it can be captured in a graph)
 hipError_t launch_fill(int *p, int n, int v, hipStream_t s);

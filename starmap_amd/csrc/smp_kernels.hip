@@ -73,8 +73,12 @@ __device__ __forceinline__ SMP_GLOBAL P *glb(P *p) {
 // batched kernel, whose work units mix substeps of different kinds).
 // AT: SubstepArgs<T> (kernel argument) or the same in the constant address
 // space (an entry of the batched kernel, read with scalar loads).
-template <typename T, int NT, int NAPP, bool GEN, typename AT>
+// CW cell waves (4 or 8): with 8, two waves per SIMD share each cell, each
+// updating half of a group's targets.
+template <typename T, int NT, int NAPP, bool GEN, int CW, typename AT>
 __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsigned char *smem_raw) {
+    constexpr int NCT = CW * 64;              // cell threads
+    constexpr int TPT = GMAX * 4 / CW;        // targets of a group per cell thread
     unsigned char *ring = smem_raw;
     T *cfa = reinterpret_cast<T *>(smem_raw + 3 * BLOB_SZ);
     T *lds = cfa + a.ncf * 256;
@@ -91,7 +95,9 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     const int fslot0 = __builtin_amdgcn_readfirstlane(a.fslot[0]);
     const int fslot1 = __builtin_amdgcn_readfirstlane(a.fslot[1]);
     const int lane = tid & 63;
-    const bool copier = tid >= 256;
+    const bool copier = tid >= NCT;
+    const int ctid = tid & 255;               // the cell of a cell thread
+    const int half = CW == 8 ? (tid >> 8) & 1 : 0;
     int b = tile;
     const int t0 = b % a.ntile[0];
     b /= a.ntile[0];
@@ -99,10 +105,10 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     const int t2 = b / a.ntile[1];
     const int o0 = t0 * a.tile[0], o1 = t1 * a.tile[1], o2 = t2 * a.tile[2];
     const int ncell = a.tile[0] * a.tile[1] * a.tile[2];
-    const bool has_cell = tid < ncell;
-    const int c0 = tid % a.tile[0];
-    const int c1 = (tid / a.tile[0]) % a.tile[1];
-    const int c2 = tid / (a.tile[0] * a.tile[1]);
+    const bool has_cell = tid < NCT && ctid < ncell;
+    const int c0 = ctid % a.tile[0];
+    const int c1 = (ctid / a.tile[0]) % a.tile[1];
+    const int c2 = ctid / (a.tile[0] * a.tile[1]);
     const int g0 = o0 + c0, g1 = o1 + c1, g2 = o2 + c2;
     const bool in_store = has_cell && g0 < a.E[0] && g1 < a.E[1] && g2 < a.E[2];
     const int lbase = c0 + a.lds[0] * (c1 + a.lds[1] * c2);
@@ -165,7 +171,7 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     }
     auto blob_dma = [&](int gq, int buf) {
         // three 1 KB chunks, 16 bytes a lane, by the first copy wave
-        if (tid >= 320) return;
+        if (tid >= NCT + 64) return;
 #pragma unroll
         for (int k = 0; k < BLOB_SZ / 1024; ++k) {
             SMP_GLOBAL const unsigned char *src = glb(a.blobs) + (size_t)gq * BLOB_SZ + k * 1024 + lane * 16;
@@ -178,7 +184,7 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
         // LDS read behind an outstanding copy would have to wait for it):
         // comp | slot << 20 | (origin-shift bits) << 28
         const int nl = __builtin_amdgcn_readfirstlane(hdr_nl(buf));
-        const int cw = (tid >> 6) - 4;   // copy wave 0 or 1: loads of its parity
+        const int cw = (tid >> 6) - CW;  // copy wave 0 or 1: loads of its parity
         int myrec = 0;
         if (lane < nl) {
             const DevLoad &d = ldrec(buf)[lane];
@@ -229,21 +235,22 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     SMP_GLOBAL char *ug_st = glb(reinterpret_cast<char *>(a.U)) + gidx * sizeof(T);
     SMP_GLOBAL T *sink = glb(a.sink);
     auto load_targets = [&](int buf, T *u) {
-        const TgtRec *r = recs(buf);
+        const TgtRec *r = recs(buf) + half * TPT;
 #pragma unroll
-        for (int i = 0; i < GMAX; ++i) u[i] = *reinterpret_cast<SMP_GLOBAL const T *>(ug_ld + r[i].uoff);
+        for (int i = 0; i < TPT; ++i) u[i] = *reinterpret_cast<SMP_GLOBAL const T *>(ug_ld + r[i].uoff);
     };
     // byte address of this thread's cell in the first source slot; term
     // offsets are byte offsets from the start of LDS
     const int lb = lbase * (int)sizeof(T);
-    // The update computes in FP64 whatever the state precision: the FP32
-    // state only halves the bytes moved, its flux sums and half-steps are
-    // not rounded to FP32 before the final store.
-    typedef double A;
+    // The update computes in the state precision (the FP32 state's flux
+    // sum and half-steps in FP32; the term coefficients of its records are
+    // FP32 too).
+    typedef T A;
+    const A hdt = (A)half_dt;
     // coefficient: staged field (off >= 0), scalar (off == -1) or, in the
     // GEN variant only, a field read from global memory (off == -2)
     auto coef = [&](int off, const void *p, double s) -> A {
-        const A staged = (A)cfa[(off > 0 ? off : 0) + tid];
+        const A staged = (A)cfa[(off > 0 ? off : 0) + ctid];
         A v = off >= 0 ? staged : (A)s;
         if (GEN && off == -2) v = in_store ? (A)glb(reinterpret_cast<const T *>(p))[gidx] : (A)0;
         return v;
@@ -252,9 +259,9 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     // (extent 0, terms in the zero tile), so the loop has no branches and the
     // compiler can overlap one target's LDS reads with another's arithmetic.
     auto update = [&](int buf, const T *u_in) {
-        const TgtRec *rb = recs(buf);
+        const TgtRec *rb = recs(buf) + half * TPT;
 #pragma unroll
-        for (int i = 0; i < GMAX; ++i) {
+        for (int i = 0; i < TPT; ++i) {
             const TgtRec &r = rb[i];
             const int4 h0 = reinterpret_cast<const int4 *>(&r)[0];   // comp filt sig_off E_off
             const int4 h1 = reinterpret_cast<const int4 *>(&r)[1];   // Q_off ext0 ext1 ext2
@@ -266,8 +273,9 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
             for (int q = 0; q < NT; ++q) {
                 const Term tm = r.t[q];
                 const unsigned char *lo = smem_raw + (tm.off + lb);
-                w += tm.coef * ((A)*reinterpret_cast<const T *>(lo + tm.stride) -
-                                (A)*reinterpret_cast<const T *>(lo));
+                const A c = sizeof(T) == 4 ? (A)tm.coef32[0] : (A)tm.coef;
+                w += c * ((A)*reinterpret_cast<const T *>(lo + tm.stride) -
+                          (A)*reinterpret_cast<const T *>(lo));
             }
             const A sig = coef(h0.z, r.sig_p, r.sig_s);
             const A E = coef(h0.w, r.E_p, r.E_s);
@@ -276,9 +284,9 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
 #pragma unroll
             for (int app = 0; app < NAPP; ++app) {
                 // U + dt/2*(W+Q - s.*U).*E   (starmap_solver.m:352-354)
-                u = u + (half_dt * (wq - sig * u)) * E;
+                u = u + (hdt * (wq - sig * u)) * E;
                 const bool s1 = (app == 0 ? fslot0 : fslot1) != 0;
-                A f = s1 ? r.F_s[1] : r.F_s[0];   // 1 for targets without a filter
+                A f = (A)(s1 ? r.F_s[1] : r.F_s[0]);   // 1 for targets without a filter
                 if (GEN && h0.y) {
                     const void *fp = s1 ? r.F[1] : r.F[0];
                     if (fp && act) f = (A)glb(reinterpret_cast<const T *>(fp))[gidx];
@@ -293,22 +301,24 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     };
 
     // ---- prologue ----------------------------------------------------------
-    T uA[GMAX], uB[GMAX];
+    T uA[TPT], uB[TPT];
     if (copier) {
         if (ng > 0) blob_dma(0, 0);
         if (ng > 1) blob_dma(1, 1);
         __builtin_amdgcn_s_waitcnt(0);
     } else {
-        // every field load in flight before the first LDS write
-        T cv[MAX_CFIELDS];
+        if (half == 0) {
+            // every field load in flight before the first LDS write
+            T cv[MAX_CFIELDS];
 #pragma unroll
-        for (int c = 0; c < MAX_CFIELDS; ++c)
-            if (c < a.ncf)
-                cv[c] = glb(reinterpret_cast<const T *>(a.cfields[c]))[in_store ? gidx : 0];
+            for (int c = 0; c < MAX_CFIELDS; ++c)
+                if (c < a.ncf)
+                    cv[c] = glb(reinterpret_cast<const T *>(a.cfields[c]))[in_store ? gidx : 0];
 #pragma unroll
-        for (int c = 0; c < MAX_CFIELDS; ++c)
-            if (c < a.ncf) cfa[c * 256 + tid] = in_store ? cv[c] : (T)0;
-        for (int e = tid; e < S; e += 256) lds[e] = (T)0;   // slot 0: the zero tile
+            for (int c = 0; c < MAX_CFIELDS; ++c)
+                if (c < a.ncf) cfa[c * 256 + ctid] = in_store ? cv[c] : (T)0;
+        }
+        for (int e = tid; e < S; e += NCT) lds[e] = (T)0;   // slot 0: the zero tile
     }
     __syncthreads();
     if (copier) {
@@ -341,20 +351,20 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     }
 }
 
-template <typename T, int NT, int NAPP, bool GEN>
-__global__ __launch_bounds__(KTHREADS) void substep_kernel(const SubstepArgs<T> a) {
+template <typename T, int NT, int NAPP, bool GEN, int CW>
+__global__ __launch_bounds__(CW * 64 + 128) void substep_kernel(const SubstepArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= a.n_tiles) return;
-    substep_tile<T, NT, NAPP, GEN>(a, tile, smem_raw);
+    substep_tile<T, NT, NAPP, GEN, CW>(a, tile, smem_raw);
 }
 
 // Batched persistent kernel: one launch runs a substep of many independent
 // problems.  Work units are (entry, tile) pairs, entries ordered by cost per
 // tile, heaviest first; every workgroup takes the next unit from a counter
 // until none is left, so uneven problems balance over the CUs.
-template <typename T, int NT, bool GEN>
-__global__ __launch_bounds__(KTHREADS) void batch_kernel(const BatchLaunch<T> *bl) {
+template <typename T, int NT, bool GEN, int CW>
+__global__ __launch_bounds__(CW * 64 + 128) void batch_kernel(const BatchLaunch<T> *bl) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int s_unit;
     const int n_units = bl->n_units;
@@ -391,9 +401,9 @@ __global__ __launch_bounds__(KTHREADS) void batch_kernel(const BatchLaunch<T> *b
         // one or two half-step applications, each compiled with its count
         // fixed (a runtime count costs the update its instruction schedule)
         if (a.napp == 2)
-            substep_tile<T, NT, 2, GEN>(a, u - ent[lo].unit0, smem_raw);
+            substep_tile<T, NT, 2, GEN, CW>(a, u - ent[lo].unit0, smem_raw);
         else
-            substep_tile<T, NT, 1, GEN>(a, u - ent[lo].unit0, smem_raw);
+            substep_tile<T, NT, 1, GEN, CW>(a, u - ent[lo].unit0, smem_raw);
     }
 }
 
@@ -402,65 +412,74 @@ __global__ void fill_kernel(int *p, int n, int v) {
     if (i < n) p[i] = v;
 }
 
-template <typename T, int NT, int NAPP, bool GEN>
-static hipError_t launch_v(const SubstepArgs<T> &a, int threads, size_t lds_bytes, hipStream_t s) {
+template <typename T, int NT, int NAPP, bool GEN, int CW>
+static hipError_t launch_v(const SubstepArgs<T> &a, size_t lds_bytes, hipStream_t s) {
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void *)substep_kernel<T, NT, NAPP, GEN>,
+        hipError_t e = hipFuncSetAttribute((const void *)substep_kernel<T, NT, NAPP, GEN, CW>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            160 * 1024);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
     const int nb = ((a.n_tiles + 7) / 8) * 8;
-    hipLaunchKernelGGL((substep_kernel<T, NT, NAPP, GEN>), dim3(nb), dim3(threads), lds_bytes, s,
-                       a);
+    hipLaunchKernelGGL((substep_kernel<T, NT, NAPP, GEN, CW>), dim3(nb), dim3(kthreads(CW)),
+                       lds_bytes, s, a);
     return hipGetLastError();
 }
 
-template <typename T, int NT>
-static hipError_t launch_nt(const SubstepArgs<T> &a, bool gen, int threads, size_t lds_bytes,
-                            hipStream_t s) {
+template <typename T, int NT, int CW>
+static hipError_t launch_nt(const SubstepArgs<T> &a, bool gen, size_t lds_bytes, hipStream_t s) {
     if (a.napp == 1)
-        return gen ? launch_v<T, NT, 1, true>(a, threads, lds_bytes, s)
-                   : launch_v<T, NT, 1, false>(a, threads, lds_bytes, s);
-    return gen ? launch_v<T, NT, 2, true>(a, threads, lds_bytes, s)
-               : launch_v<T, NT, 2, false>(a, threads, lds_bytes, s);
+        return gen ? launch_v<T, NT, 1, true, CW>(a, lds_bytes, s)
+                   : launch_v<T, NT, 1, false, CW>(a, lds_bytes, s);
+    return gen ? launch_v<T, NT, 2, true, CW>(a, lds_bytes, s)
+               : launch_v<T, NT, 2, false, CW>(a, lds_bytes, s);
 }
 
 template <typename T>
-hipError_t launch_substep(const SubstepArgs<T> &a, int nt, bool gen, int threads,
-                          size_t lds_bytes, hipStream_t s) {
-    if (a.ncf > MAX_CFIELDS || threads != KTHREADS || a.chunks > CPS_MAX ||
+hipError_t launch_substep(const SubstepArgs<T> &a, int nt, bool gen, int cw, size_t lds_bytes,
+                          hipStream_t s) {
+    if (a.ncf > MAX_CFIELDS || (cw != 4 && cw != 8) || a.chunks > CPS_MAX ||
         (a.napp != 1 && a.napp != 2))
         return hipErrorInvalidValue;
-    if (nt <= 8) return launch_nt<T, 8>(a, gen, threads, lds_bytes, s);
-    return launch_nt<T, MAXT>(a, gen, threads, lds_bytes, s);
+    if (cw == 8) return nt <= 8 ? launch_nt<T, 8, 8>(a, gen, lds_bytes, s)
+                                : launch_nt<T, MAXT, 8>(a, gen, lds_bytes, s);
+    return nt <= 8 ? launch_nt<T, 8, 4>(a, gen, lds_bytes, s)
+                   : launch_nt<T, MAXT, 4>(a, gen, lds_bytes, s);
 }
 
 // (the dynamic LDS limit of batch_kernel is raised by batch_occupancy, which
 // runs before any launch and outside graph capture)
-template <typename T>
-static const void *batch_fn(int nt, bool gen) {
-    if (nt <= 8) return gen ? (const void *)batch_kernel<T, 8, true> : (const void *)batch_kernel<T, 8, false>;
-    return gen ? (const void *)batch_kernel<T, MAXT, true> : (const void *)batch_kernel<T, MAXT, false>;
+template <typename T, int CW>
+static const void *batch_fn_cw(int nt, bool gen) {
+    if (nt <= 8)
+        return gen ? (const void *)batch_kernel<T, 8, true, CW> : (const void *)batch_kernel<T, 8, false, CW>;
+    return gen ? (const void *)batch_kernel<T, MAXT, true, CW>
+               : (const void *)batch_kernel<T, MAXT, false, CW>;
 }
 
 template <typename T>
-hipError_t launch_batch(const BatchLaunch<T> *bl, int nt, bool gen, int grid, size_t lds_bytes,
-                        hipStream_t s) {
-    if (grid < 1 || lds_bytes > 160 * 1024 - 64) return hipErrorInvalidValue;
-    const void *f = batch_fn<T>(nt, gen);
+static const void *batch_fn(int nt, bool gen, int cw) {
+    return cw == 8 ? batch_fn_cw<T, 8>(nt, gen) : batch_fn_cw<T, 4>(nt, gen);
+}
+
+template <typename T>
+hipError_t launch_batch(const BatchLaunch<T> *bl, int nt, bool gen, int cw, int grid,
+                        size_t lds_bytes, hipStream_t s) {
+    if (grid < 1 || lds_bytes > 160 * 1024 - 64 || (cw != 4 && cw != 8)) return hipErrorInvalidValue;
+    const void *f = batch_fn<T>(nt, gen, cw);
     void *args[] = {(void *)&bl};
-    return hipLaunchKernel(f, dim3(grid), dim3(KTHREADS), args, lds_bytes, s);
+    return hipLaunchKernel(f, dim3(grid), dim3(kthreads(cw)), args, lds_bytes, s);
 }
 
 template <typename T>
-hipError_t batch_occupancy(int nt, bool gen, size_t lds_bytes, int *blocks) {
-    const void *f = batch_fn<T>(nt, gen);
+hipError_t batch_occupancy(int nt, bool gen, int cw, size_t lds_bytes, int *blocks) {
+    if (cw != 4 && cw != 8) return hipErrorInvalidValue;
+    const void *f = batch_fn<T>(nt, gen, cw);
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64);
     if (e != hipSuccess) return e;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, f, KTHREADS, lds_bytes);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, f, kthreads(cw), lds_bytes);
 }
 
 hipError_t launch_fill(int *p, int n, int v, hipStream_t s) {
@@ -623,9 +642,9 @@ hipError_t launch_reduce_final(const double *part, int blocks, double *out3, hip
                                          hipStream_t);                                         \
     template hipError_t launch_reduce<T>(const T *, const ReduceGeom &, double *, int,         \
                                          hipStream_t);                                         \
-    template hipError_t launch_batch<T>(const BatchLaunch<T> *, int, bool, int, size_t,        \
+    template hipError_t launch_batch<T>(const BatchLaunch<T> *, int, bool, int, int, size_t,   \
                                         hipStream_t);                                          \
-    template hipError_t batch_occupancy<T>(int, bool, size_t, int *);
+    template hipError_t batch_occupancy<T>(int, bool, int, size_t, int *);
 
 SMP_INSTANTIATE(double)
 SMP_INSTANTIATE(float)

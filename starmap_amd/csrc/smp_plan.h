@@ -36,7 +36,10 @@ struct DevLoad {
 struct alignas(16) Term {
     int32_t off;        // offset of the low sample (slot base + shifts)
     int32_t stride;     // stride of the coupled axis
-    double coef;        // -M(j,k)/h; 0 for padding terms
+    union {
+        double coef;        // -M(j,k)/h; 0 for padding terms
+        float coef32[2];    // the same rounded to FP32 (FP32-state device copy)
+    };
 };
 
 // One target of a substep: its coupling terms and its coefficients.  The

@@ -58,10 +58,11 @@ def _as_coef(v, shape):
 class HipSolver:
     """One StaRMAP problem on one GPU (a ``smp_problem`` handle)."""
 
-    def __init__(self, par, precision="fp64", device=0, tile=None, blocks_per_cu=0):
+    def __init__(self, par, precision="fp64", device=0, tile=None, blocks_per_cu=0, cell_waves=0):
         """tile: cells per workgroup tile per kernel axis (None: library
         default); blocks_per_cu: workgroups per CU the LDS schedule is made
-        for (0: library default)."""
+        for, cell_waves: 4 or 8 waves updating the cells (0: library
+        defaults)."""
         L = _lib.lib()
         par = dict(par)
         Mx, My, Mz = (np.asarray(par[k], dtype=np.float64) for k in ("Mx", "My", "Mz"))
@@ -127,6 +128,7 @@ class HipSolver:
             for e, v in enumerate(tile):
                 d.tile[e] = int(v)
         d.blocks_per_cu = int(blocks_per_cu)
+        d.cell_waves = int(cell_waves)
         self.precision = precision
         h = ctypes.c_void_p()
         _lib.check(L.smp_create(ctypes.byref(d), ctypes.byref(h)))
@@ -214,19 +216,22 @@ class HipSolver:
                                               ctypes.byref(bad)))
         return s, q, bool(bad.value)
 
-    def set_schedule(self, tile=None, blocks_per_cu=0):
+    def set_schedule(self, tile=None, blocks_per_cu=0, cell_waves=0):
         """Re-make the substep schedules for another tile shape / LDS
-        budget; state and coefficients are kept."""
+        budget / cell-wave count; state and coefficients are kept."""
         t = np.zeros(3, dtype=np.int32)
         if tile is not None:
             t[:len(tile)] = tile
-        _lib.check(_lib.lib().smp_set_schedule(self.handle, t.ctypes.data, int(blocks_per_cu)))
+        _lib.check(_lib.lib().smp_set_schedule(self.handle, t.ctypes.data, int(blocks_per_cu),
+                                               int(cell_waves)))
 
     def get_schedule(self):
+        """(tile, blocks_per_cu, cell_waves)"""
         t = np.zeros(3, dtype=np.int32)
-        b = ctypes.c_int32()
-        _lib.check(_lib.lib().smp_get_schedule(self.handle, t.ctypes.data, ctypes.byref(b)))
-        return tuple(int(v) for v in t), b.value
+        b, c = ctypes.c_int32(), ctypes.c_int32()
+        _lib.check(_lib.lib().smp_get_schedule(self.handle, t.ctypes.data, ctypes.byref(b),
+                                               ctypes.byref(c)))
+        return tuple(int(v) for v in t), b.value, c.value
 
     def checkpoint_save(self, slot):
         _lib.check(_lib.lib().smp_checkpoint_save(self.handle, int(slot)))

@@ -51,14 +51,14 @@ def single_runs(pars, precision):
     return out
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("precision,cw", [("fp32", 4), ("fp64", 4), ("fp32", 8)])
 @pytest.mark.parametrize("use_graph", [False, True])
-def test_batch_bitwise_equal_single_handles(precision, use_graph):
+def test_batch_bitwise_equal_single_handles(precision, cw, use_graph):
     pars = small_sweep()
-    print("batch", precision, "graph" if use_graph else "eager",
+    print("batch", precision, cw, "graph" if use_graph else "eager",
           [(p["n"][1], p["n_mom"]) for p in pars], flush=True)
     ref = single_runs(pars, precision)
-    b = BatchSolver(pars, precision=precision)
+    b = BatchSolver(pars, precision=precision, cell_waves=cw)
     try:
         for rep in range(2):          # the second run replays the kept graph
             dts, ns = b.prepare()
@@ -96,14 +96,14 @@ def test_schedule_change_keeps_every_bit():
     prob = P.lattice_2d(n=60, n_mom=9, tfinal=0.3)
     par = I.starmap_init(prob)
     res = []
-    for tile, bpc in [(None, 0), ((127, 2), 1), ((31, 8), 2)]:
+    for tile, bpc, cw in [(None, 0, 0), ((127, 2), 1, 8), ((31, 8), 2, 4), ((63, 4), 1, 8)]:
         s = HipSolver(par, precision="fp64")
         try:
             dt = s.prepare()
             s.step(dt, 2)
             if tile is not None:
-                s.set_schedule(tile, bpc)
-                assert s.get_schedule() == (tuple(tile) + (1,), bpc)
+                s.set_schedule(tile, bpc, cw)
+                assert s.get_schedule() == (tuple(tile) + (1,), bpc, cw)
             s.step(dt, 5)
             res.append(states(s))
         finally:
@@ -126,8 +126,8 @@ def test_checkpoint_round_trip_and_autotune_keep_state():
         s.step(dt, 3)
         s.checkpoint_load(3)
         assert all(np.array_equal(x, y) for x, y in zip(a, states(s)))
-        tile, bpc, times = autotune.tune(s, steps=1, use_cache=False)
-        assert times and s.get_schedule() == (tuple(tile), bpc)
+        best, times = autotune.tune(s, steps=1, use_cache=False)
+        assert times and s.get_schedule() == best
         assert all(np.array_equal(x, y) for x, y in zip(a, states(s)))
         s.step(dt, 3)
         r.step(dt, 5)
@@ -144,8 +144,8 @@ def test_batch_autotune():
     ref = single_runs(pars, "fp32")
     try:
         dts, ns = b.prepare()
-        tile, bpc, times = autotune.tune(b, steps=1, use_cache=False)
-        assert times
+        best, times = autotune.tune(b, steps=1, use_cache=False)
+        assert times and all(x.get_schedule() == best for x in b.solvers)
         b.step(ns)
         for i, s in enumerate(b.solvers):
             assert all(np.array_equal(x, y) for x, y in zip(states(s), ref[i]))
