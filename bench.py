@@ -12,7 +12,7 @@ Prints ONE JSON line: value = FP64 moment-cell updates/s
 (mcu/s = n_sys * cells * steps / wall), the roofline of the substep kernel
 (algorithmic bytes per launch / mean launch time from HIP events on the
 library's stream), the FP32-state variant with its error against the FP64
-state after the same steps, and a CPU baseline (the FP64 oracle on the host
+state after the same steps (each on the schedule the autotuner picks), and a CPU baseline (the FP64 oracle on the host
 cores, a bounded sample of the same workload).  A second leg ("gauss3d")
 measures the 3D P_7 (64 moments) 200^3 periodic Gaussian problem in FP64 and
 FP32 state; a third ("batched") 64 seeded 2D problems of uneven size, order
@@ -56,11 +56,17 @@ def sync():
         pass
 
 
-def run_gpu(prob, precision, steps, warmup):
-    from starmap_amd import init
+def run_gpu(prob, precision, steps, warmup, tune=True):
+    """Set up, pick the schedule with the autotuner (state restored after
+    the trials), warm up, then time `steps` steps."""
+    from starmap_amd import autotune, init
     from starmap_amd.solver import HipSolver
     s = HipSolver(init.starmap_init(prob), precision=precision)
     dt = s.prepare()
+    if tune:
+        (tile, bpc, cw), _ = autotune.tune(s, steps=2, use_cache=False)
+        s.schedule = {"tile": list(tile), "blocks_per_cu": bpc, "cell_waves": cw}
+        log("  autotuner: tile %s, %d workgroups/CU budget, %d cell waves" % (tile, bpc, cw))
     if warmup:
         s.step(dt, warmup)
     sync()
@@ -126,6 +132,7 @@ def gauss3d_leg(n, n_mom, steps, warmup):
         log("3d %s: setting up %d^3 P%d" % (prec, n, n_mom))
         s, dt, wall, _ = run_gpu(prob, prec, steps, warmup)
         leg[prec] = kernel_roofline(s, steps, cells, wall, elem)
+        leg[prec]["schedule"] = getattr(s, "schedule", None)
         log("3d %s: %d steps in %.3f s" % (prec, steps, wall))
         ss[prec] = s
     leg["fp32"]["speedup_vs_fp64"] = leg["fp32"]["value"] / leg["fp64"]["value"]
@@ -369,6 +376,7 @@ def main():
             "n_sys": n_sys, "cells": cells, "grid": [args.n, args.n], "bc": "extrapolation",
             "filter": "exp p=4 f_eff=10 substep", "dt_over_dx": 0.5,
             "absorber_blocks": 12, "seed": 2024,
+            "schedule": getattr(s, "schedule", None),
         },
         "roofline": {
             "bound": "hbm",
@@ -398,6 +406,7 @@ def main():
             "speedup_vs_fp64": (mcu / wall32) / out["value"],
             "roofline_achieved_GBs": ach32,
             "roofline_frac": ach32 / HBM_PEAK_GBS,
+            "schedule": getattr(s32, "schedule", None),
         }, **err)
     for v in res.values():
         v[0].close()

@@ -86,7 +86,7 @@ typedef struct smp_desc {
     int32_t cell_waves;       /* waves updating the cells of a tile: 4, or 8
                                  (two per SIMD, each with half of a group's
                                  targets); 0 takes SMP_CELL_WAVES from the
-                                 environment, else 8 for FP32 and 4 for FP64 */
+                                 environment, else 8                          */
 } smp_desc;
 
 /* Last error message of the calling thread ("" if none). */

@@ -374,8 +374,7 @@ int smp_create(const smp_desc *d, smp_problem **out) {
     int want_blocks = 1;
     if (const char *env = std::getenv("SMP_BLOCKS_PER_CU")) want_blocks = std::atoi(env);
     if (d->blocks_per_cu) want_blocks = d->blocks_per_cu;
-    // FP32 fits two cell waves per SIMD in registers; FP64 would spill
-    int cw = d->precision == SMP_FP32 ? 8 : 4;
+    int cw = 8;
     if (const char *env = std::getenv("SMP_CELL_WAVES")) cw = std::atoi(env);
     if (d->cell_waves) cw = d->cell_waves;
     if (cw != 4 && cw != 8) return fail(SMP_E_INVALID, "cell_waves must be 4 or 8");

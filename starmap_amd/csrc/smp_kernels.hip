@@ -95,7 +95,9 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     const int fslot0 = __builtin_amdgcn_readfirstlane(a.fslot[0]);
     const int fslot1 = __builtin_amdgcn_readfirstlane(a.fslot[1]);
     const int lane = tid & 63;
-    const bool copier = tid >= NCT;
+    // wave-uniform (waves are 64 threads, NCT a multiple of 64): copy waves
+    // and cell waves run separate loops with the same number of barriers
+    const bool copier = __builtin_amdgcn_readfirstlane(tid) >= NCT;
     const int ctid = tid & 255;               // the cell of a cell thread
     const int half = CW == 8 ? (tid >> 8) & 1 : 0;
     int b = tile;
@@ -300,53 +302,55 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
         }
     };
 
-    // ---- prologue ----------------------------------------------------------
-    T uA[TPT], uB[TPT];
+    // ---- copy waves ----------------------------------------------------------
+    // prologue (two barriers), then one barrier per group: the sources of
+    // group g+1 and the records of group g+2 while the cells update group g
     if (copier) {
         if (ng > 0) blob_dma(0, 0);
         if (ng > 1) blob_dma(1, 1);
         __builtin_amdgcn_s_waitcnt(0);
-    } else {
-        if (half == 0) {
-            // every field load in flight before the first LDS write
-            T cv[MAX_CFIELDS];
-#pragma unroll
-            for (int c = 0; c < MAX_CFIELDS; ++c)
-                if (c < a.ncf)
-                    cv[c] = glb(reinterpret_cast<const T *>(a.cfields[c]))[in_store ? gidx : 0];
-#pragma unroll
-            for (int c = 0; c < MAX_CFIELDS; ++c)
-                if (c < a.ncf) cfa[c * 256 + ctid] = in_store ? cv[c] : (T)0;
-        }
-        for (int e = tid; e < S; e += NCT) lds[e] = (T)0;   // slot 0: the zero tile
-    }
-    __syncthreads();
-    if (copier) {
+        __syncthreads();
         if (ng > 0) src_dma(0);
         __builtin_amdgcn_s_waitcnt(0);
-    } else if (ng > 0) {
-        load_targets(0, uA);
-    }
-    __syncthreads();
-
-    // ---- groups ------------------------------------------------------------
-    // one barrier phase per group; the cell waves load the target values of
-    // the next group after storing this group's, into the other of two
-    // register sets (no register copies between phases): the loads complete
-    // while the copy waves finish, and the next phase's first read of them is
-    // the only wait for memory in the cell waves
-    for (int gq = 0; gq < ng; ++gq) {
-        if (copier) {
+        __syncthreads();
+        for (int gq = 0; gq < ng; ++gq) {
             if (gq + 1 < ng) src_dma((gq + 1) % 3);
             if (gq + 2 < ng) blob_dma(gq + 2, (gq + 2) % 3);
             __builtin_amdgcn_s_waitcnt(0);
-        } else if (gq & 1) {
-            update(gq % 3, uB);
-            if (gq + 1 < ng) load_targets((gq + 1) % 3, uA);
-        } else {
-            update(gq % 3, uA);
-            if (gq + 1 < ng) load_targets((gq + 1) % 3, uB);
+            __syncthreads();
         }
+        return;
+    }
+
+    // ---- cell waves ----------------------------------------------------------
+    // the same barriers; after updating group g, the target values of group
+    // g+1 are loaded into the other of two register sets (the loop is unrolled
+    // by two, so no values move between sets): they arrive while the copy
+    // waves finish, and the next phase's first read of them is the only wait
+    // for memory in the cell waves
+    T uA[TPT], uB[TPT];
+    if (half == 0) {
+        // every field load in flight before the first LDS write
+        T cv[MAX_CFIELDS];
+#pragma unroll
+        for (int c = 0; c < MAX_CFIELDS; ++c)
+            if (c < a.ncf)
+                cv[c] = glb(reinterpret_cast<const T *>(a.cfields[c]))[in_store ? gidx : 0];
+#pragma unroll
+        for (int c = 0; c < MAX_CFIELDS; ++c)
+            if (c < a.ncf) cfa[c * 256 + ctid] = in_store ? cv[c] : (T)0;
+    }
+    for (int e = tid; e < S; e += NCT) lds[e] = (T)0;   // slot 0: the zero tile
+    __syncthreads();
+    if (ng > 0) load_targets(0, uA);
+    __syncthreads();
+    for (int gq = 0; gq < ng; gq += 2) {
+        update(gq % 3, uA);
+        if (gq + 1 < ng) load_targets((gq + 1) % 3, uB);
+        __syncthreads();
+        if (gq + 1 >= ng) break;
+        update((gq + 1) % 3, uB);
+        if (gq + 2 < ng) load_targets((gq + 2) % 3, uA);
         __syncthreads();
     }
 }

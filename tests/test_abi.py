@@ -137,7 +137,7 @@ def test_batch_and_schedule_host_checks():
         assert L.smp_batch_create(arr, 1, ctypes.byref(b)) == _lib.SMP_E_STATE
         assert L.smp_batch_create(arr, 0, ctypes.byref(b)) == _lib.SMP_E_INVALID
         assert L.smp_batch_create(None, 1, ctypes.byref(b)) == _lib.SMP_E_INVALID
-        assert s.get_schedule() == ((63, 4, 1), 1, 4)       # FP64 default: 4 cell waves
+        assert s.get_schedule() == ((63, 4, 1), 1, 8)
         before = [s.plan_stats(k) for k in (0, 1)]
         s.set_schedule((127, 2), 2, 8)
         assert s.get_schedule() == ((127, 2, 1), 2, 8)
