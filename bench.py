@@ -56,13 +56,18 @@ def sync():
         pass
 
 
-def run_gpu(prob, precision, steps, warmup, tune=True):
+TUNE = True
+
+
+def run_gpu(prob, precision, steps, warmup, tune=None):
     """Set up, pick the schedule with the autotuner (state restored after
     the trials), warm up, then time `steps` steps."""
     from starmap_amd import autotune, init
     from starmap_amd.solver import HipSolver
     s = HipSolver(init.starmap_init(prob), precision=precision)
     dt = s.prepare()
+    if tune is None:
+        tune = TUNE
     if tune:
         (tile, bpc, cw), _ = autotune.tune(s, steps=2, use_cache=False)
         s.schedule = {"tile": list(tile), "blocks_per_cu": bpc, "cell_waves": cw}
@@ -314,7 +319,11 @@ def main():
                     help="steps of the adjoint leg (0 skips it)")
     ap.add_argument("--fd-cells", type=int, default=8,
                     help="cells of the adjoint leg's finite-difference check")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="library default schedules instead of the autotuner's (profiling runs)")
     args = ap.parse_args()
+    global TUNE
+    TUNE = not args.no_tune
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the time-stepping core is a single-GPU "
               "kernel path (--gpus %d requested)" % args.gpus, file=sys.stderr)
