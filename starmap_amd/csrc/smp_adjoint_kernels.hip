@@ -27,15 +27,6 @@ __device__ __forceinline__ SMP_GLOBAL P *glb(P *p) {
 
 namespace {
 
-__device__ __forceinline__ int wrapc(int i, int n, int bc, int ext) {
-    if (bc == 0) {
-        if (i < 0) i += n;
-        if (i >= n) i -= n;
-        return i;
-    }
-    return i < 0 ? 0 : (i >= ext ? ext - 1 : i);
-}
-
 __device__ __forceinline__ double expm1div_d(double x, double *dedx) {
     if (fabs(x) > 2e-4) {
         const double ex = exp(x);
@@ -46,62 +37,209 @@ __device__ __forceinline__ double expm1div_d(double x, double *dedx) {
     return 1 + x * .5 + (x * x) / 6;
 }
 
-}  // namespace
+// index i+d on an axis: wrapped on periodic axes, clamped into [0, ext)
+// on extrapolation axes
+__device__ __forceinline__ int adj_index(int i, int d, int n, int bc, int ext) {
+    int j = i + d;
+    if (bc == 0) return j < 0 ? j + n : (j >= n ? j - n : j);
+    return j < 0 ? 0 : (j >= ext ? ext - 1 : j);
+}
+
+// element at byte offset `off` (< 4 GiB) of the slab of component k
+template <typename T>
+__device__ __forceinline__ T slab_at(const T *base, int64_t slab_bytes, int k, uint32_t off) {
+    const SMP_GLOBAL char *b = glb(reinterpret_cast<const char *>(base)) + k * slab_bytes;
+    return *reinterpret_cast<const SMP_GLOBAL T *>(b + off);
+}
+
+// Offsets (elements) of the high and low samples of the forward flux of a
+// target at index i of an axis: a cell target (tcell) reads faces i+1 and
+// i, a face target reads cells i and i-1 (starmap_solver.m:352-354 and the
+// boundary extension of :359-377)
+__device__ __forceinline__ void row_offsets(int i, bool tcell, int n, int bc, int stride, int &oh,
+                                            int &ol) {
+    const int ext = tcell ? n + bc : n;
+    const int dh = tcell ? 1 : 0, dl = tcell ? 0 : -1;
+    oh = (adj_index(i, dh, n, bc, ext) - i) * stride;
+    ol = (adj_index(i, dl, n, bc, ext) - i) * stride;
+}
+
+// Transposed: a cell component at index i is the high sample of face
+// target i and the low sample of face target i+1 (targets in [1, n) count
+// on extrapolation axes, the boundary faces carry no difference); a face
+// component is the high sample of cell target i-1 and the low sample of
+// cell target i
+__device__ __forceinline__ void col_offsets(int i, bool kcell, int n, int bc, int stride, int &oh,
+                                            int &ol, bool &mh, bool &ml) {
+    const int ext = kcell ? n + bc : n;
+    const int dh = kcell ? 0 : -1, dl = kcell ? 1 : 0, lb = kcell ? 1 : 0;
+    const int ih = i + dh, il = i + dl;
+    mh = bc == 0 || (ih >= lb && ih < n);
+    ml = bc == 0 || (il >= lb && il < n);
+    oh = (adj_index(i, dh, n, bc, ext) - i) * stride;
+    ol = (adj_index(i, dl, n, bc, ext) - i) * stride;
+}
+
+// Point rows of a thread: 64 consecutive i0 per wave, the row index (i1, i2)
+// the same across the wave (made explicit for the compiler)
+__device__ __forceinline__ int wave_row(int row) { return __builtin_amdgcn_readfirstlane(row); }
+
+// Sample offsets of a point on each kernel axis (per lane on axis 0, the
+// same across the wave on axes 1 and 2) and, for transposed terms, whether
+// each sample counts.
+struct AxisOffsets {
+    int h[3], l[3];
+    bool mh[3], ml[3];
+};
+
+// One group of terms: coefficients and the high and low samples at element
+// pe + offsets of the group's axis.
+template <typename T>
+struct Group {
+    T vh[ADJ_GROUP], vl[ADJ_GROUP];
+    double coef[ADJ_GROUP];
+};
 
 template <typename T>
+__device__ __forceinline__ void gather(Group<T> &gr, const T *base, int64_t sb,
+                                       const SMP_GLOBAL AdjTerm *g, int pe, int oh, int ol) {
+    int k[ADJ_GROUP];
+#pragma unroll
+    for (int t = 0; t < ADJ_GROUP; ++t) {
+        k[t] = g[t].k;
+        gr.coef[t] = g[t].coef;
+    }
+    const uint32_t bh = (uint32_t)(pe + oh) * sizeof(T), bl = (uint32_t)(pe + ol) * sizeof(T);
+#pragma unroll
+    for (int t = 0; t < ADJ_GROUP; ++t) {
+        gr.vh[t] = slab_at(base, sb, k[t], bh);
+        gr.vl[t] = slab_at(base, sb, k[t], bl);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ double reduce(const Group<T> &gr, bool mh, bool ml) {
+    double w = 0;
+#pragma unroll
+    for (int t = 0; t < ADJ_GROUP; ++t) {
+        double d = 0;
+        if (mh) d += (double)gr.vh[t];
+        if (ml) d -= (double)gr.vl[t];
+        w += gr.coef[t] * d;
+    }
+    return w;
+}
+
+// sum over a component's term list of coef*(high - low) (masked for
+// transposed terms), axes in order: the first group of every axis gathered
+// by gather_first (the caller issues every first group before reducing any)
+template <typename T, int NDIM>
+__device__ __forceinline__ void gather_first(Group<T> *g0, const T *base, int64_t sb,
+                                             const SMP_GLOBAL AdjTerm *terms, const int32_t *ra, int pe,
+                                             const AxisOffsets &o) {
+#pragma unroll
+    for (int e = 0; e < NDIM; ++e) gather(g0[e], base, sb, terms + ra[e], pe, o.h[e], o.l[e]);
+}
+
+template <typename T, int NDIM>
+__device__ __forceinline__ double flux_rest(const Group<T> *g0, const T *base, int64_t sb,
+                                            const SMP_GLOBAL AdjTerm *terms, const int32_t *ra, int pe,
+                                            const AxisOffsets &o) {
+    double w = 0;
+#pragma unroll
+    for (int e = 0; e < NDIM; ++e) {
+        w += reduce(g0[e], o.mh[e], o.ml[e]);
+        for (int m = ra[e] + ADJ_GROUP; m < ra[e + 1]; m += ADJ_GROUP) {
+            Group<T> gr;
+            gather(gr, base, sb, terms + m, pe, o.h[e], o.l[e]);
+            w += reduce(gr, o.mh[e], o.ml[e]);
+        }
+    }
+    return w;
+}
+
+}  // namespace
+
+// Adjoint of one launch for the targets of one grid-type group per item.
+// FUSE: first lambda += the transposed flux of the launch after it (its mu,
+// the other parity, terms by source column), as adj_transpose_kernel stores
+// it.
+template <typename T, bool FUSE, int NDIM>
 __global__ __launch_bounds__(256) void adj_update_kernel(const AdjArgs<T> a) {
-    const int grp = blockIdx.y;
-    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int i0 = (int)(t % a.E[0]);
-    const int64_t r = t / a.E[0];
-    const int i1 = (int)(r % a.E[1]);
-    const int i2 = (int)(r / a.E[1]);
+    // Work order: grid-type groups of the same points next to each other
+    // (they share source components), then 64-point chunks along i0, then
+    // groups of four rows; the hardware deals workgroups to the 8 XCDs in
+    // turn, so workgroup b takes the item that gives each XCD (and its L2)
+    // one contiguous band of rows
+    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+    const int xcd = b & 7, per = nb >> 3, rem = nb & 7;
+    const int item = xcd * per + (xcd < rem ? xcd : rem) + (b >> 3);
+    const int grp = item % a.n_grp;
+    const int nchunk = (a.E[0] + 63) >> 6;
+    const int rest = item / a.n_grp;
+    const int row = wave_row((rest / nchunk) * 4 + (int)threadIdx.y);
+    if (row >= a.E[1] * a.E[2]) return;
     const int q0 = a.grp0[grp], q1 = a.grp0[grp + 1];
     const SMP_GLOBAL AdjComp<T> *comp = glb(a.comp);
     const SMP_GLOBAL int32_t *list = glb(a.list);
-    {
-        const int jf = list[q0];
-        if (i2 >= a.E[2] || i0 >= comp[jf].ext[0] || i1 >= comp[jf].ext[1] || i2 >= comp[jf].ext[2])
-            return;
+    const int jf = list[q0];
+    int i[3];
+    i[0] = (rest % nchunk) * 64 + (int)threadIdx.x;
+    i[1] = row % a.E[1];
+    i[2] = row / a.E[1];
+    if (i[0] >= comp[jf].ext[0] || i[1] >= comp[jf].ext[1] || i[2] >= comp[jf].ext[2]) return;
+    const int pe = i[0] + a.pitch0 * (i[1] + a.ext1 * i[2]);
+    // sample offsets of the group's grid type
+    const int stride[3] = {1, a.pitch0, a.pitch0 * a.ext1};
+    AxisOffsets ro, co;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const bool cl = comp[jf].cell[e] != 0;
+        row_offsets(i[e], cl, a.n[e], a.bc[e], stride[e], ro.h[e], ro.l[e]);
+        ro.mh[e] = ro.ml[e] = true;
+        if (FUSE) col_offsets(i[e], cl, a.n[e], a.bc[e], stride[e], co.h[e], co.l[e], co.mh[e], co.ml[e]);
     }
-    const int ii[3] = {i0, i1, i2};
-    const int64_t p = i0 + (int64_t)a.pitch0 * (i1 + (int64_t)a.ext1 * i2);
     const double hd = a.half_dt;
-    const SMP_GLOBAL AdjTerm *rows = glb(a.rows);
+    const int64_t sb = a.slab * (int64_t)sizeof(T);
+    const uint32_t pb = (uint32_t)pe * sizeof(T);
     double ga = 0, gs = 0;
     for (int q = q0; q < q1; ++q) {
         const int j = list[q];
         const SMP_GLOBAL AdjComp<T> &c = comp[j];
-        // W of the forward step (x, y, z terms in the forward order)
-        double w = 0;
-        for (int m = c.r0; m < c.r1; ++m) {
-            const SMP_GLOBAL AdjTerm &tm = rows[m];
-            const int e = tm.axis;
-            int hi[3] = {ii[0], ii[1], ii[2]}, lo[3] = {ii[0], ii[1], ii[2]};
-            if (comp[tm.k].cell[e])
-                lo[e] = ii[e] - 1;   // face target between cells i-1 and i
-            else
-                hi[e] = ii[e] + 1;   // cell target between faces i and i+1
-            const int ext = comp[tm.k].ext[e];
-            hi[e] = wrapc(hi[e], a.n[e], a.bc[e], ext);
-            lo[e] = wrapc(lo[e], a.n[e], a.bc[e], ext);
-            const SMP_GLOBAL T *Uk = glb(a.U) + (int64_t)tm.k * a.slab;
-            const double vh = (double)Uk[hi[0] + (int64_t)a.pitch0 * (hi[1] + (int64_t)a.ext1 * hi[2])];
-            const double vl = (double)Uk[lo[0] + (int64_t)a.pitch0 * (lo[1] + (int64_t)a.ext1 * lo[2])];
-            w += tm.coef * (vh - vl);
+        int32_t ra[4], ca[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            ra[e] = c.ra[e];
+            ca[e] = c.ca[e];
         }
-        const double sg = c.sig ? (double)glb(c.sig)[p] : c.sig_s;
-        const double E = c.E ? (double)glb(c.E)[p] : c.E_s;
-        double dedx;
-        (void)expm1div_d(-sg * hd, &dedx);
-        const double dE = -hd * dedx;
-        const double wq = w + (c.Q ? (double)glb(c.Q)[p] : 0.0);
-        const int64_t jp = (int64_t)j * a.slab + p;
+        // every load of the component (the first group of each term list
+        // on each axis included) issued before the first use: fields that
+        // are scalars read U0 instead
+        const bool has_sig = c.sig != nullptr, has_E = c.E != nullptr, has_Q = c.Q != nullptr;
+        const T u0_t = slab_at(a.U0, sb, j, pb);
+        const T lam_t = slab_at(a.lam, sb, j, pb);
+        const T sig_t = slab_at(has_sig ? c.sig : a.U0, 0, 0, pb);
+        const bool has_dE = c.dE != nullptr;
+        const T E_t = slab_at(has_E ? c.E : a.U0, 0, 0, pb);
+        const T dE_t = slab_at(has_dE ? c.dE : a.U0, 0, 0, pb);
+        const T Q_t = slab_at(has_Q ? c.Q : a.U0, 0, 0, pb);
+        Group<T> gr[NDIM], gc[NDIM];
+        gather_first<T, NDIM>(gr, a.U, sb, glb(a.rows), ra, pe, ro);
+        if (FUSE) gather_first<T, NDIM>(gc, a.mu, sb, glb(a.cols), ca, pe, co);
+        // W of the forward step (terms in the forward order)
+        const double w = flux_rest<T, NDIM>(gr, a.U, sb, glb(a.rows), ra, pe, ro);
+        double lam = (double)lam_t;
+        // lambda of the launch after this one, as the transposed-flux
+        // kernel stores it
+        if (FUSE) lam = (double)(T)(lam + flux_rest<T, NDIM>(gc, a.mu, sb, glb(a.cols), ca, pe, co));
+        const double sg = has_sig ? (double)sig_t : c.sig_s;
+        const double E = has_E ? (double)E_t : c.E_s;
+        const double wq = w + (has_Q ? (double)Q_t : 0.0);
+        const double dE = has_dE ? (double)dE_t : c.dE_s;
         // states before each application (the forward rounds to T after each)
         double u[2];
-        u[0] = (double)glb(a.U0)[jp];
+        u[0] = (double)u0_t;
         u[1] = (double)(T)((u[0] + (hd * (wq - sg * u[0])) * E) * c.F[a.fslot[0] ? 1 : 0]);
-        double lam = (double)glb(a.lam)[jp];
         double mu = 0, g = 0;
         for (int app = a.napp - 1; app >= 0; --app) {
             const double l = c.F[a.fslot[app] ? 1 : 0] * lam;
@@ -109,67 +247,86 @@ __global__ __launch_bounds__(256) void adj_update_kernel(const AdjArgs<T> a) {
             g += l * (-hd * u[app] * E + hd * (wq - sg * u[app]) * dE);
             lam = (1 - hd * sg * E) * l;
         }
+        const int64_t jp = (int64_t)j * a.slab + pe;
         glb(a.lam)[jp] = (T)lam;
         glb(a.mu)[jp] = (T)mu;
         ga += g;
         if (j > 0) gs += g;
     }
-    const int64_t gp = (int64_t)a.gcode[grp] * a.slab + p;
+    const int64_t gp = (int64_t)a.gcode[grp] * a.slab + pe;
     glb(a.Ga)[gp] += ga;
     glb(a.Gs)[gp] += gs;
 }
 
-template <typename T>
+// lambda_k += the transposed flux of mu for the components k in a.list
+// (grid.z), ADJ_ROWS rows of points per thread
+template <typename T, int NDIM>
 __global__ __launch_bounds__(256) void adj_transpose_kernel(const AdjArgs<T> a) {
-    const int k = glb(a.list)[blockIdx.y];
+    const int k = glb(a.list)[blockIdx.z];
     const SMP_GLOBAL AdjComp<T> &c = glb(a.comp)[k];
-    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int x0 = (int)(t % a.E[0]);
-    const int64_t r = t / a.E[0];
-    const int x1 = (int)(r % a.E[1]);
-    const int x2 = (int)(r / a.E[1]);
-    if (x2 >= a.E[2] || x0 >= c.ext[0] || x1 >= c.ext[1] || x2 >= c.ext[2]) return;
-    const int xx[3] = {x0, x1, x2};
-    double acc = 0;
-    for (int q = c.c0; q < c.c1; ++q) {
-        const SMP_GLOBAL AdjTerm &tm = glb(a.cols)[q];
-        const int e = tm.axis;
-        const int n = a.n[e];
-        const SMP_GLOBAL T *mj = glb(a.mu) + (int64_t)tm.k * a.slab;
-        // target indices whose high / low sample is x, and whether they count
-        int ih[3] = {xx[0], xx[1], xx[2]}, il[3] = {xx[0], xx[1], xx[2]};
-        bool use_h, use_l;
-        if (c.cell[e]) {
-            // face target s reads cells s (high) and s-1 (low)
-            ih[e] = xx[e];
-            il[e] = xx[e] + 1;
-            if (a.bc[e] == 0) {
-                il[e] = il[e] == n ? 0 : il[e];
-                use_h = use_l = true;
-            } else {
-                // boundary faces 0 and n have zero differences
-                use_h = ih[e] >= 1 && ih[e] <= n - 1;
-                use_l = il[e] >= 1 && il[e] <= n - 1;
-            }
-        } else {
-            // cell target i reads faces i+1 (high) and i (low)
-            ih[e] = xx[e] - 1;
-            il[e] = xx[e];
-            if (a.bc[e] == 0) {
-                ih[e] = ih[e] < 0 ? n - 1 : ih[e];
-                use_h = use_l = true;
-            } else {
-                use_h = ih[e] >= 0;
-                use_l = il[e] <= n - 1;
-            }
-        }
-        double d = 0;
-        if (use_h) d += (double)mj[ih[0] + (int64_t)a.pitch0 * (ih[1] + (int64_t)a.ext1 * ih[2])];
-        if (use_l) d -= (double)mj[il[0] + (int64_t)a.pitch0 * (il[1] + (int64_t)a.ext1 * il[2])];
-        acc += tm.coef * d;
+    const int ext0 = c.ext[0], ext1 = c.ext[1], ext2 = c.ext[2];
+    const int i0 = (int)blockIdx.y * 64 + (int)threadIdx.x;
+    if (i0 >= ext0) return;
+    int32_t ca[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) ca[e] = c.ca[e];
+    const int64_t sb = a.slab * (int64_t)sizeof(T);
+    const int stride[3] = {1, a.pitch0, a.pitch0 * a.ext1};
+    // rows outside the component read its last row and store nothing
+    const int nrows = a.E[1] * a.E[2];
+    const int row0 = wave_row(((int)blockIdx.x * 4 + (int)threadIdx.y) * ADJ_ROWS);
+    int pe[ADJ_ROWS];
+    bool live[ADJ_ROWS];
+    AxisOffsets co[ADJ_ROWS];
+    T lam0[ADJ_ROWS];
+#pragma unroll
+    for (int r = 0; r < ADJ_ROWS; ++r) {
+        const int row = row0 + r;
+        int i[3];
+        i[0] = i0;
+        i[1] = row % a.E[1];
+        i[2] = row / a.E[1];
+        live[r] = row < nrows && i[1] < ext1 && i[2] < ext2;
+        i[1] = i[1] < ext1 ? i[1] : ext1 - 1;
+        i[2] = i[2] < ext2 ? i[2] : ext2 - 1;
+        pe[r] = i[0] + a.pitch0 * (i[1] + a.ext1 * i[2]);
+#pragma unroll
+        for (int e = 0; e < 3; ++e)
+            col_offsets(i[e], c.cell[e] != 0, a.n[e], a.bc[e], stride[e], co[r].h[e], co[r].l[e],
+                        co[r].mh[e], co[r].ml[e]);
+        lam0[r] = slab_at(a.lam, sb, k, (uint32_t)pe[r] * sizeof(T));
     }
-    const int64_t kp = (int64_t)k * a.slab + x0 + (int64_t)a.pitch0 * (x1 + (int64_t)a.ext1 * x2);
-    glb(a.lam)[kp] = (T)((double)glb(a.lam)[kp] + acc);
+    Group<T> g0[ADJ_ROWS][NDIM];
+#pragma unroll
+    for (int r = 0; r < ADJ_ROWS; ++r) gather_first<T, NDIM>(g0[r], a.mu, sb, glb(a.cols), ca, pe[r], co[r]);
+    double acc[ADJ_ROWS];
+#pragma unroll
+    for (int r = 0; r < ADJ_ROWS; ++r)
+        acc[r] = flux_rest<T, NDIM>(g0[r], a.mu, sb, glb(a.cols), ca, pe[r], co[r]);
+#pragma unroll
+    for (int r = 0; r < ADJ_ROWS; ++r)
+        if (live[r]) glb(a.lam)[(int64_t)k * a.slab + pe[r]] = (T)((double)lam0[r] + acc[r]);
+}
+
+// dE/dsigma of E = expm1div(-sigma*dt/2): -dt/2 * expm1div'(-sigma*dt/2)
+template <typename T>
+__global__ void dedsig_kernel(const T *sig, T *dE, int64_t count, double dt) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double dedx;
+        (void)expm1div_d(-(double)glb(sig)[i] * (dt / 2), &dedx);
+        glb(dE)[i] = (T)(-(dt / 2) * dedx);
+    }
+}
+
+// dst = src on the slabs of the components in list (elements [0, slab) of
+// each): grid.y over the list
+template <typename T>
+__global__ void copy_list_kernel(T *dst, const T *src, const int32_t *list, int64_t slab) {
+    const int64_t base = (int64_t)glb(list)[blockIdx.y] * slab;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < slab;
+         i += (int64_t)gridDim.x * blockDim.x)
+        glb(dst)[base + i] = glb(src)[base + i];
 }
 
 template <typename T>
@@ -180,23 +337,63 @@ __global__ void copy_kernel(T *dst, const T *src, int64_t count) {
 }
 
 template <typename T>
-static dim3 adj_grid(const AdjArgs<T> &a, int y) {
-    const int64_t cells = (int64_t)a.E[0] * a.E[1] * a.E[2];
-    return dim3((unsigned)((cells + 255) / 256), (unsigned)y);
+static dim3 adj_grid(const AdjArgs<T> &a, int z, int rows_per_thread) {
+    const int64_t rows = (int64_t)a.E[1] * a.E[2];
+    const int64_t per_block = 4 * rows_per_thread;
+    return dim3((unsigned)((rows + per_block - 1) / per_block), (unsigned)((a.E[0] + 63) / 64),
+                (unsigned)z);
 }
 
 template <typename T>
-hipError_t launch_adj_update(const AdjArgs<T> &a, hipStream_t s) {
+hipError_t launch_adj_update(const AdjArgs<T> &a, bool fuse, hipStream_t s) {
     if (a.n_list <= 0) return hipSuccess;
     if (a.napp != 1 && a.napp != 2) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(adj_update_kernel<T>, adj_grid(a, a.n_grp), dim3(256), 0, s, a);
+    const int64_t rows = (int64_t)a.E[1] * a.E[2];
+    const int64_t items = (rows + 3) / 4 * ((a.E[0] + 63) / 64) * a.n_grp;
+    if (items >= (int64_t)1 << 31) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)items), block(64, 4);
+#define SMP_ADJ_UPD(F, D) hipLaunchKernelGGL((adj_update_kernel<T, F, D>), grid, block, 0, s, a)
+    switch (a.ndim * 2 + (fuse ? 1 : 0)) {
+    case 2: SMP_ADJ_UPD(false, 1); break;
+    case 3: SMP_ADJ_UPD(true, 1); break;
+    case 4: SMP_ADJ_UPD(false, 2); break;
+    case 5: SMP_ADJ_UPD(true, 2); break;
+    case 6: SMP_ADJ_UPD(false, 3); break;
+    case 7: SMP_ADJ_UPD(true, 3); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef SMP_ADJ_UPD
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_adj_transpose(const AdjArgs<T> &a, hipStream_t s) {
     if (a.n_list <= 0) return hipSuccess;
-    hipLaunchKernelGGL(adj_transpose_kernel<T>, adj_grid(a, a.n_list), dim3(256), 0, s, a);
+    const dim3 grid = adj_grid(a, a.n_list, ADJ_ROWS), block(64, 4);
+    switch (a.ndim) {
+    case 1: hipLaunchKernelGGL((adj_transpose_kernel<T, 1>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((adj_transpose_kernel<T, 2>), grid, block, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((adj_transpose_kernel<T, 3>), grid, block, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_dedsig(const T *sig, T *dE, int64_t count, double dt, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((count + 255) / 256, 4096);
+    hipLaunchKernelGGL(dedsig_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, s, sig, dE, count, dt);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_copy_list(T *dst, const T *src, const int32_t *list, int n_list, int64_t slab,
+                            hipStream_t s) {
+    if (n_list <= 0 || slab <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((slab + 255) / 256, 1024);
+    hipLaunchKernelGGL(copy_list_kernel<T>, dim3((unsigned)blocks, (unsigned)n_list), dim3(256), 0, s,
+                       dst, src, list, slab);
     return hipGetLastError();
 }
 
@@ -209,9 +406,11 @@ hipError_t launch_copy(T *dst, const T *src, int64_t count, hipStream_t s) {
 }
 
 #define SMP_ADJ_INSTANTIATE(T)                                                     \
-    template hipError_t launch_adj_update<T>(const AdjArgs<T> &, hipStream_t);     \
+    template hipError_t launch_adj_update<T>(const AdjArgs<T> &, bool, hipStream_t); \
     template hipError_t launch_adj_transpose<T>(const AdjArgs<T> &, hipStream_t);  \
-    template hipError_t launch_copy<T>(T *, const T *, int64_t, hipStream_t);
+    template hipError_t launch_copy<T>(T *, const T *, int64_t, hipStream_t);    \
+    template hipError_t launch_dedsig<T>(const T *, T *, int64_t, double, hipStream_t); \
+    template hipError_t launch_copy_list<T>(T *, const T *, const int32_t *, int, int64_t, hipStream_t);
 
 SMP_ADJ_INSTANTIATE(double)
 SMP_ADJ_INSTANTIATE(float)

@@ -113,6 +113,7 @@ struct smp_problem {
     int32_t adj_gcode[2][8] = {};
     void *adj_comp = nullptr, *adj_rows = nullptr, *adj_cols = nullptr, *adj_list = nullptr;
     std::vector<void *> adj_seg, adj_rec;
+    std::map<const void *, void *> adj_dE;   // dE/dsigma field of each sigma field
     double adj_fwd_ms = 0, adj_rev_ms = 0;
     double last_total_ms = 0, last_substep_ms = 0, last_bytes = 0;
     int64_t last_launches = 0;
@@ -884,14 +885,14 @@ double substep_bytes(const smp_problem *h, int kind) {
 // device tables of the adjoint kernels (coefficients as the forward uses
 // them at the current dt)
 template <typename T>
-int adjoint_tables(smp_problem *h) {
+int adjoint_tables(smp_problem *h, double dt) {
     const smp::Geometry &g = h->geo;
+    std::set<const void *> fresh;   // dE/dsigma fields computed for this dt
     const int n = g.n_sys;
     int kaxis_of[3] = {-1, -1, -1};
     for (int e = 0; e < g.ndim; ++e) kaxis_of[g.axis[e]] = e;
     std::vector<smp::AdjComp<T>> comp(n);
     std::vector<smp::AdjTerm> rows, cols;
-    std::vector<std::vector<smp::AdjTerm>> by_src(n);
     for (int j = 0; j < n; ++j) {
         smp::AdjComp<T> &c = comp[j];
         std::memset(&c, 0, sizeof(c));
@@ -911,6 +912,19 @@ int adjoint_tables(smp_problem *h) {
         c.sig_s = sig->scalar;
         c.E = (const T *)E->dev;
         c.E_s = E->scalar;
+        if (sig->dev) {
+            void *&d = h->adj_dE[sig->dev];
+            if (!d) {
+                int rc = h->dalloc(&d, (size_t)h->slab * sizeof(T));
+                if (rc) return rc;
+            }
+            if (fresh.insert(sig->dev).second)
+                HIP_TRY(smp::launch_dedsig<T>((const T *)sig->dev, (T *)d, h->slab, dt, h->stream));
+            c.dE = (const T *)d;
+        } else {
+            const double x = -sig->scalar * dt / 2;
+            c.dE_s = -dt / 2 * (std::fabs(x) > 2e-4 ? (std::exp(x) * (x - 1) + 1) / (x * x) : .5 + x / 3);
+        }
         c.Q = h->Q[j].set ? (const T *)h->Q[j].dev : nullptr;
         if (h->Q[j].set && !h->Q[j].dev && h->Q[j].scalar != 0.0)
             return fail(SMP_E_INVALID, "the adjoint needs sources as fields");
@@ -930,27 +944,49 @@ int adjoint_tables(smp_problem *h) {
             c.ext[e] = e < g.ndim ? g.ext_of(j, g.axis[e]) : 1;
             c.cell[e] = e < g.ndim && g.gt[3 * j + g.axis[e]] == 1;
         }
-        c.r0 = (int32_t)rows.size();
+    }
+    // terms of each component by kernel axis, each active axis padded to
+    // whole groups (at least one) with coefficient 0 terms that sample the
+    // axis list's first component, or the list's own component when the
+    // axis list is empty
+    std::vector<std::vector<std::vector<smp::AdjTerm>>> rows_by(n), cols_by(n);
+    for (int j = 0; j < n; ++j) {
+        rows_by[j].resize(g.ndim);
+        cols_by[j].resize(g.ndim);
+    }
+    for (int j = 0; j < n; ++j)
         for (int d = 0; d < 3; ++d) {
             const smp::Csr &M = g.M[d];
             if (M.rp.empty()) continue;
+            const int e = kaxis_of[d];
             for (int q = M.rp[j]; q < M.rp[j + 1]; ++q) {
+                const double coef = -M.cv[q] / g.h[d];
                 smp::AdjTerm t;
+                std::memset(&t, 0, sizeof(t));
                 t.k = M.ci[q];
-                t.axis = kaxis_of[d];
-                t.coef = -M.cv[q] / g.h[d];
-                rows.push_back(t);
-                smp::AdjTerm u = t;
-                u.k = j;
-                by_src[M.ci[q]].push_back(u);
+                t.coef = coef;
+                rows_by[j][e].push_back(t);
+                t.k = j;
+                cols_by[M.ci[q]][e].push_back(t);
             }
         }
-        c.r1 = (int32_t)rows.size();
-    }
-    for (int k = 0; k < n; ++k) {
-        comp[k].c0 = (int32_t)cols.size();
-        cols.insert(cols.end(), by_src[k].begin(), by_src[k].end());
-        comp[k].c1 = (int32_t)cols.size();
+    auto flatten = [&](std::vector<std::vector<smp::AdjTerm>> &by, std::vector<smp::AdjTerm> &out,
+                       int own, int32_t *ra) {
+        for (int e = 0; e < 4; ++e) ra[e] = (int32_t)out.size();
+        for (int e = 0; e < g.ndim; ++e) {
+            ra[e] = (int32_t)out.size();
+            std::vector<smp::AdjTerm> &v = by[e];
+            smp::AdjTerm pad;
+            std::memset(&pad, 0, sizeof(pad));
+            pad.k = v.empty() ? own : v[0].k;
+            while (v.empty() || v.size() % smp::ADJ_GROUP) v.push_back(pad);
+            out.insert(out.end(), v.begin(), v.end());
+        }
+        for (int e = g.ndim; e < 4; ++e) ra[e] = (int32_t)out.size();
+    };
+    for (int j = 0; j < n; ++j) {
+        flatten(rows_by[j], rows, j, comp[j].ra);
+        flatten(cols_by[j], cols, j, comp[j].ca);
     }
     // components of each parity grouped by grid type: odd first, then even
     std::vector<int32_t> list;
@@ -1016,6 +1052,9 @@ template <typename T>
 int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
     if (h->f_position == SMP_FILTER_FULL)
         return fail(SMP_E_INVALID, "the adjoint supports no filter or the substep filter");
+    // the adjoint kernels address a component with 32-bit byte offsets
+    if ((double)h->slab * h->elem >= 2147483648.0)
+        return fail(SMP_E_INVALID, "the adjoint supports components of less than 2 GiB");
     int rc = adjoint_buffers(h);
     if (rc) return rc;
     // the forward coefficients at this dt
@@ -1027,7 +1066,7 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
         rc = refresh_coefs<T>(h);
         if (rc) return rc;
     }
-    rc = adjoint_tables<T>(h);
+    rc = adjoint_tables<T>(h, dt);
     if (rc) return rc;
     const size_t st = (size_t)h->slab * h->elem * h->geo.n_sys;
     const int64_t n_seg = (n_steps + every - 1) / every;
@@ -1039,10 +1078,14 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
         if (rc) return rc;
         h->adj_seg.push_back(p);
     }
-    while ((int64_t)h->adj_rec.size() < 2 * (int64_t)every) {
+    // records of a segment's recomputation: the targets of every launch
+    // before it (2*every+1 launches); the other parity's slabs of a record
+    // are never written and stay zero
+    while ((int64_t)h->adj_rec.size() < 2 * (int64_t)every + 1) {
         void *p = nullptr;
         rc = h->dalloc(&p, st);
         if (rc) return rc;
+        HIP_TRY(hipMemsetAsync(p, 0, st, h->stream));
         h->adj_rec.push_back(p);
     }
     hipEvent_t e0, e1, e2;
@@ -1092,43 +1135,54 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
         a.napp = napp;
         a.fslot[0] = f0;
         a.fslot[1] = f1;
+        a.ndim = h->geo.ndim;
         return a;
     };
-    // reverse sweep, one segment at a time: recompute it from its checkpoint
-    // recording the states each adjoint substep needs, then step back
+    // reverse sweep, one segment at a time.  The segment's launches are the
+    // forward's (odd first, then even and fused odd per step, odd last):
+    // launch L updates the targets of one parity from the other parity with
+    // napp applications.  The recomputation records the targets of each
+    // launch before it; the sources of launch L are the targets of L+1
+    // before L+1 (the current state for the last launch).  The adjoint
+    // steps back launch by launch: the update of L's targets, then the
+    // transposed flux into L's sources.
+    const smp::SubstepArgs<T> f_odd_first = make_args<T>(h, 0, dt, 1, 0, 0, h->work);
+    const smp::SubstepArgs<T> f_even = make_args<T>(h, 1, dt, 2, 0, 1, h->work);
+    const smp::SubstepArgs<T> f_odd_fused = make_args<T>(h, 0, dt, 2, 1, 0, h->work);
+    const smp::SubstepArgs<T> f_odd_last = make_args<T>(h, 0, dt, 1, 1, 1, h->work);
     for (int64_t sg = n_seg - 1; sg >= 0; --sg) {
         const int64_t ns = std::min<int64_t>(every, n_steps - sg * every);
+        const int64_t nl = 2 * ns + 1;
+        auto launch_of = [&](int64_t L) -> const smp::SubstepArgs<T> & {
+            if (L == 0) return f_odd_first;
+            if (L == nl - 1) return f_odd_last;
+            return L % 2 ? f_even : f_odd_fused;
+        };
         HIP_TRY(hipMemcpyAsync(h->work, h->adj_seg[sg], st, hipMemcpyDeviceToDevice, h->stream));
-        const smp::SubstepArgs<T> s1 = make_args<T>(h, 0, dt, 1, 0, 0, h->work);
-        const smp::SubstepArgs<T> s2 = make_args<T>(h, 1, dt, 2, 0, 1, h->work);
-        const smp::SubstepArgs<T> s3 = make_args<T>(h, 0, dt, 1, 1, 1, h->work);
-        for (int64_t s = 0; s < ns; ++s) {
-            HIP_TRY(hipMemcpyAsync(h->adj_rec[2 * s], h->work, st, hipMemcpyDeviceToDevice, h->stream));
-            HIP_TRY(smp::launch_substep<T>(s1, h->plan[0].max_terms, h->gen[0], h->cell_waves,
-                                           h->lds_bytes[0], h->stream));
-            HIP_TRY(smp::launch_substep<T>(s2, h->plan[1].max_terms, h->gen[1], h->cell_waves,
-                                           h->lds_bytes[1], h->stream));
-            HIP_TRY(hipMemcpyAsync(h->adj_rec[2 * s + 1], h->work, st, hipMemcpyDeviceToDevice,
-                                   h->stream));
-            if (s + 1 < ns)
-                HIP_TRY(smp::launch_substep<T>(s3, h->plan[0].max_terms, h->gen[0], h->cell_waves,
-                                               h->lds_bytes[0], h->stream));
+        for (int64_t L = 0; L < nl; ++L) {
+            const int kind = L % 2;   // 0: odd targets
+            HIP_TRY(smp::launch_copy_list<T>((T *)h->adj_rec[L], (const T *)h->work,
+                                             kind ? list_even : list_odd, kind ? nl_even : nl_odd,
+                                             h->slab, h->stream));
+            HIP_TRY(smp::launch_substep<T>(launch_of(L), h->plan[kind].max_terms, h->gen[kind],
+                                           h->cell_waves, h->lds_bytes[kind], h->stream));
         }
-        for (int64_t s = ns - 1; s >= 0; --s) {
-            const void *S0 = h->adj_rec[2 * s], *S1 = h->adj_rec[2 * s + 1];
-            // substep 3: odd from S1's odd, sources S1's even; filter slot k=2
-            HIP_TRY(smp::launch_adj_update<T>(adj_args(S1, S1, list_odd, nl_odd, 1, 1, 1), h->stream));
-            HIP_TRY(smp::launch_adj_transpose<T>(adj_args(S1, S1, list_even, nl_even, 1, 1, 1),
-                                                 h->stream));
-            // substep 2: even from S0's even, sources S1's odd; two
-            // applications, slots k=1 then k=2
-            HIP_TRY(smp::launch_adj_update<T>(adj_args(S1, S0, list_even, nl_even, 2, 0, 1), h->stream));
-            HIP_TRY(smp::launch_adj_transpose<T>(adj_args(S1, S0, list_odd, nl_odd, 2, 0, 1),
-                                                 h->stream));
-            // substep 1: odd from S0's odd, sources S0's even; slot k=1
-            HIP_TRY(smp::launch_adj_update<T>(adj_args(S0, S0, list_odd, nl_odd, 1, 0, 0), h->stream));
-            HIP_TRY(smp::launch_adj_transpose<T>(adj_args(S0, S0, list_even, nl_even, 1, 0, 0),
-                                                 h->stream));
+        // the transposed flux of launch L+1 is fused into the update of L
+        // (both touch lambda of L's targets); the segment's first launch
+        // ends with it as a launch of its own
+        for (int64_t L = nl - 1; L >= 0; --L) {
+            const int kind = L % 2;
+            const smp::SubstepArgs<T> &f = launch_of(L);
+            const void *U0 = h->adj_rec[L];
+            const void *U = L + 1 < nl ? h->adj_rec[L + 1] : h->work;
+            const int32_t *tl = kind ? list_even : list_odd, *sl = kind ? list_odd : list_even;
+            const int ntl = kind ? nl_even : nl_odd, nsl = kind ? nl_odd : nl_even;
+            HIP_TRY(smp::launch_adj_update<T>(adj_args(U, U0, tl, ntl, f.napp, f.fslot[0], f.fslot[1]),
+                                              L + 1 < nl, h->stream));
+            if (L == 0)
+                HIP_TRY(smp::launch_adj_transpose<T>(adj_args(U, U0, sl, nsl, f.napp, f.fslot[0],
+                                                              f.fslot[1]),
+                                                     h->stream));
         }
     }
     HIP_TRY(hipEventRecord(e2, h->stream));

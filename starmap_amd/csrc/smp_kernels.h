@@ -118,13 +118,21 @@ hipError_t launch_reduce_final(const double *part, int blocks, double *out3, hip
 namespace smp {
 
 // ---- discrete adjoint of the time step ------------------------------------
-// One coupling term of the adjoint tables: component `k` along kernel axis
-// `axis` with coefficient -M(j,k)/h (the forward kernel's coefficient).
+// One coupling term of the adjoint tables: component `k` with coefficient
+// -M(j,k)/h (the forward kernel's coefficient).  A component's terms are
+// listed by kernel axis (the forward's order), each active axis's list
+// holding at least one group of ADJ_GROUP terms, padded with coef = 0 terms.
+// Every component coupled to another along an axis has the opposite
+// staggering on that axis, so the sample offsets follow from the axis and
+// the grid type of the component that owns the list.
 struct AdjTerm {
     int32_t k;
-    int32_t axis;
+    int32_t pad;
     double coef;
 };
+constexpr int ADJ_GROUP = 4;
+// rows of points per thread in the transposed-flux kernel
+constexpr int ADJ_ROWS = 2;
 
 // Per component: its coefficients in the forward half-step, its extents and
 // grid types per kernel axis and its ranges in the term tables (terms by
@@ -133,13 +141,14 @@ template <typename T>
 struct AdjComp {
     const T *sig;             // sigma field (slab layout) or null: sig_s
     const T *E;               // expm1div(-sigma dt/2) field or null: E_s
+    const T *dE;              // dE/dsigma field or null: dE_s
     const T *Q;               // source field or null (no source)
-    double sig_s, E_s;
+    double sig_s, E_s, dE_s;
     double F[2];              // filter factors of slots k=1,2 (1 without filter)
     int32_t ext[3];
     int32_t cell[3];          // 1 if cell-centred on the kernel axis
-    int32_t r0, r1;           // forward terms (this component as target)
-    int32_t c0, c1;           // transposed terms (this component as source)
+    int32_t ra[4];            // forward terms on kernel axis e: [ra[e], ra[e+1])
+    int32_t ca[4];            // transposed terms (this component as source)
 };
 
 template <typename T>
@@ -165,14 +174,22 @@ struct AdjArgs {
     double half_dt;
     int32_t napp;
     int32_t fslot[2];
+    int32_t ndim;             // active kernel axes (1..3)
 };
 
 // adjoint of the half-step applications of one substep for the targets in
 // a.list: mu = B*lambda', lambda = A*lambda', and per grid type the sum over
 // its targets of lambda'*(dA u + dB (W+Q)) added to Ga (all targets) and Gs
 // (targets j > 1): one thread per point and grid type
+// dE[i] = dE/dsigma of expm1div(-sigma*dt/2) at sig[i]
 template <typename T>
-hipError_t launch_adj_update(const AdjArgs<T> &a, hipStream_t s);
+hipError_t launch_dedsig(const T *sig, T *dE, int64_t count, double dt, hipStream_t s);
+
+// fuse: first lambda += the transposed flux of mu (the other parity's
+// targets of the launch after this one), as launch_adj_transpose for the
+// sources of that launch
+template <typename T>
+hipError_t launch_adj_update(const AdjArgs<T> &a, bool fuse, hipStream_t s);
 
 // lambda_k += sum_j coef_jk (D^T mu_j), k in a.list (the substep's sources)
 template <typename T>
@@ -181,5 +198,11 @@ hipError_t launch_adj_transpose(const AdjArgs<T> &a, hipStream_t s);
 // dst[i] = src[i] for a contiguous range of elements
 template <typename T>
 hipError_t launch_copy(T *dst, const T *src, int64_t count, hipStream_t s);
+
+// the slabs of the components in list (device array of n_list indices)
+// copied from src to dst
+template <typename T>
+hipError_t launch_copy_list(T *dst, const T *src, const int32_t *list, int n_list, int64_t slab,
+                            hipStream_t s);
 
 }  // namespace smp

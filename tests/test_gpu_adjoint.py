@@ -177,3 +177,47 @@ def test_torch_autograd_function():
         s.close()
     assert np.allclose(ta.grad.numpy().ravel(), ga, rtol=1e-12, atol=0)
     assert np.allclose(ts.grad.numpy().ravel(), gs, rtol=1e-12, atol=0)
+
+
+def small_3d(n=(6, 5, 4), n_mom=3, steps=5, bc=(1, 0, 1)):
+    """3D P_N with mixed boundaries, spatially varying sigma_a, sigma_s0 and
+    source (fields on every staggered grid)."""
+    lam = P.legendre_max_root(n_mom + 1)
+    h = 2.0 / max(n)
+    return {
+        "name": "adjoint 3D", "closure": "P", "n_mom": n_mom,
+        "sigma_a": (lambda x, y, z: 0.4 + 0.2 * np.sin(3 * x + 1) * np.cos(2 * y) + 0.1 * z),
+        "sigma_s0": (lambda x, y, z: 1.0 + 0.3 * np.cos(x * y + z)),
+        "source": (lambda x, y, z: np.exp(-((x + 0.3) ** 2 + y ** 2 + (z - 0.2) ** 2) / 0.2)),
+        "ax": [-1.0, 1.0, -1.0, 1.0, -1.0, 1.0], "n": list(n), "bc": list(bc),
+        "cn": 1.5 * lam, "t_plot": [], "tfinal": steps * 0.5 * h, "mom_output": 1, "output": None,
+    }
+
+
+def test_lambda0_equals_finite_difference_3d():
+    """The transposed stencil on three axes, periodic and extrapolation
+    boundaries mixed: lambda(0) against central differences of the FP64
+    forward at corners and random points of components of both parities."""
+    prob = small_3d()
+    s = Adjoint(I.starmap_init(prob), precision="fp64")
+    try:
+        rng = np.random.default_rng(3)
+        s.prepare()
+        mask = rng.random(s.shape(0))
+        s.run({0: mask}, every=2)
+        for j in (0, 1, 2, 3, 7):
+            lam = s.lam(j)
+            shp = lam.shape
+            corners = [tuple(c * (d - 1) for c, d in zip(cs, shp))
+                       for cs in ((0, 0, 0), (1, 1, 1), (1, 0, 1), (0, 1, 0))]
+            pts = corners + [tuple(int(rng.integers(d)) for d in shp) for _ in range(3)]
+            for p in pts:
+                e = np.zeros(shp)
+                e[p] = 1.0
+                Jp = forward_J(prob, mask, state={j: e})
+                Jm = forward_J(prob, mask, state={j: -e})
+                fd = (Jp - Jm) / 2
+                print("lambda3d", j, p, lam[p], fd, flush=True)
+                assert abs(lam[p] - fd) <= 1e-9 * max(abs(fd), np.abs(lam).max()), (j, p, lam[p], fd)
+    finally:
+        s.close()
