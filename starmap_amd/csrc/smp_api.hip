@@ -107,7 +107,8 @@ struct smp_problem {
     // discrete adjoint: lambda, mu (state layout), dJ/dsigma per component
     // (FP64), tables of the adjoint kernels, segment checkpoints and the
     // states recorded while a segment is recomputed
-    void *lam = nullptr, *mu = nullptr, *G = nullptr, *work = nullptr;
+    void *lam = nullptr, *mu = nullptr, *G = nullptr, *work = nullptr, *work2 = nullptr;
+    hipStream_t adj_stream = nullptr;   // the adjoint's recomputation (default priority)
     int adj_ngrp[2] = {0, 0};     // odd, even: grid types, list ranges, codes
     int32_t adj_grp0[2][9] = {};
     int32_t adj_gcode[2][8] = {};
@@ -121,8 +122,10 @@ struct smp_problem {
     std::vector<void *> allocs;
 
     ~smp_problem() {
+        if (adj_stream) (void)hipStreamSynchronize(adj_stream);
         for (void *p : allocs) (void)hipFree(p);
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        if (adj_stream) (void)hipStreamDestroy(adj_stream);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -1031,7 +1034,7 @@ int adjoint_tables(smp_problem *h, double dt) {
 
 int adjoint_buffers(smp_problem *h) {
     const size_t st = (size_t)h->slab * h->elem * h->geo.n_sys;
-    for (void **p : {&h->lam, &h->mu, &h->work}) {
+    for (void **p : {&h->lam, &h->mu, &h->work, &h->work2}) {
         if (!*p) {
             int rc = h->dalloc(p, st);
             if (rc) return rc;
@@ -1078,10 +1081,12 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
         if (rc) return rc;
         h->adj_seg.push_back(p);
     }
-    // records of a segment's recomputation: the targets of every launch
-    // before it (2*every+1 launches); the other parity's slabs of a record
-    // are never written and stay zero
-    while ((int64_t)h->adj_rec.size() < 2 * (int64_t)every + 1) {
+    // records of two segments' recomputations (the next segment is
+    // recomputed while the current one is stepped back): the targets of
+    // every launch before it (2*every+1 launches); the other parity's slabs
+    // of a record are never written and stay zero
+    const int64_t rec_per_set = 2 * (int64_t)every + 1;
+    while ((int64_t)h->adj_rec.size() < 2 * rec_per_set) {
         void *p = nullptr;
         rc = h->dalloc(&p, st);
         if (rc) return rc;
@@ -1146,35 +1151,70 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
     // before L+1 (the current state for the last launch).  The adjoint
     // steps back launch by launch: the update of L's targets, then the
     // transposed flux into L's sources.
-    const smp::SubstepArgs<T> f_odd_first = make_args<T>(h, 0, dt, 1, 0, 0, h->work);
-    const smp::SubstepArgs<T> f_even = make_args<T>(h, 1, dt, 2, 0, 1, h->work);
-    const smp::SubstepArgs<T> f_odd_fused = make_args<T>(h, 0, dt, 2, 1, 0, h->work);
-    const smp::SubstepArgs<T> f_odd_last = make_args<T>(h, 0, dt, 1, 1, 1, h->work);
-    for (int64_t sg = n_seg - 1; sg >= 0; --sg) {
-        const int64_t ns = std::min<int64_t>(every, n_steps - sg * every);
-        const int64_t nl = 2 * ns + 1;
-        auto launch_of = [&](int64_t L) -> const smp::SubstepArgs<T> & {
-            if (L == 0) return f_odd_first;
-            if (L == nl - 1) return f_odd_last;
-            return L % 2 ? f_even : f_odd_fused;
-        };
-        HIP_TRY(hipMemcpyAsync(h->work, h->adj_seg[sg], st, hipMemcpyDeviceToDevice, h->stream));
+    // Two buffer sets (records and a running state): segment sg uses set
+    // sg % 2.  The recomputation runs on its own stream one segment ahead
+    // of the reverse sweep, which runs on the handle's stream.
+    if (!h->adj_stream) HIP_TRY(hipStreamCreateWithFlags(&h->adj_stream, hipStreamNonBlocking));
+    hipStream_t rs = h->adj_stream;
+    hipEvent_t ev_fwd, ev_rc[2], ev_rev[2];
+    HIP_TRY(hipEventCreateWithFlags(&ev_fwd, hipEventDisableTiming));
+    for (int q = 0; q < 2; ++q) {
+        HIP_TRY(hipEventCreateWithFlags(&ev_rc[q], hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&ev_rev[q], hipEventDisableTiming));
+    }
+    HIP_TRY(hipEventRecord(ev_fwd, h->stream));
+    HIP_TRY(hipStreamWaitEvent(rs, ev_fwd, 0));
+    void *const work_of[2] = {h->work, h->work2};
+    smp::SubstepArgs<T> f_odd_first[2], f_even[2], f_odd_fused[2], f_odd_last[2];
+    for (int q = 0; q < 2; ++q) {
+        f_odd_first[q] = make_args<T>(h, 0, dt, 1, 0, 0, work_of[q]);
+        f_even[q] = make_args<T>(h, 1, dt, 2, 0, 1, work_of[q]);
+        f_odd_fused[q] = make_args<T>(h, 0, dt, 2, 1, 0, work_of[q]);
+        f_odd_last[q] = make_args<T>(h, 0, dt, 1, 1, 1, work_of[q]);
+    }
+    auto seg_launches = [&](int64_t sg) { return 2 * std::min<int64_t>(every, n_steps - sg * every) + 1; };
+    auto launch_of = [&](int q, int64_t L, int64_t nl) -> const smp::SubstepArgs<T> & {
+        if (L == 0) return f_odd_first[q];
+        if (L == nl - 1) return f_odd_last[q];
+        return L % 2 ? f_even[q] : f_odd_fused[q];
+    };
+    auto rec = [&](int q, int64_t L) { return h->adj_rec[q * rec_per_set + L]; };
+    // recompute segment sg on the recomputation stream
+    auto recompute = [&](int64_t sg) -> int {
+        const int q = (int)(sg % 2);
+        const int64_t nl = seg_launches(sg);
+        HIP_TRY(hipMemcpyAsync(work_of[q], h->adj_seg[sg], st, hipMemcpyDeviceToDevice, rs));
         for (int64_t L = 0; L < nl; ++L) {
             const int kind = L % 2;   // 0: odd targets
-            HIP_TRY(smp::launch_copy_list<T>((T *)h->adj_rec[L], (const T *)h->work,
+            HIP_TRY(smp::launch_copy_list<T>((T *)rec(q, L), (const T *)work_of[q],
                                              kind ? list_even : list_odd, kind ? nl_even : nl_odd,
-                                             h->slab, h->stream));
-            HIP_TRY(smp::launch_substep<T>(launch_of(L), h->plan[kind].max_terms, h->gen[kind],
-                                           h->cell_waves, h->lds_bytes[kind], h->stream));
+                                             h->slab, rs));
+            HIP_TRY(smp::launch_substep<T>(launch_of(q, L, nl), h->plan[kind].max_terms, h->gen[kind],
+                                           h->cell_waves, h->lds_bytes[kind], rs));
         }
+        HIP_TRY(hipEventRecord(ev_rc[q], rs));
+        return SMP_OK;
+    };
+    rc = recompute(n_seg - 1);
+    if (rc) return rc;
+    for (int64_t sg = n_seg - 1; sg >= 0; --sg) {
+        const int q = (int)(sg % 2);
+        const int64_t nl = seg_launches(sg);
+        // the next segment's recomputation reuses the set of segment sg+1
+        if (sg >= 1) {
+            if (sg + 1 < n_seg) HIP_TRY(hipStreamWaitEvent(rs, ev_rev[(sg + 1) % 2], 0));
+            rc = recompute(sg - 1);
+            if (rc) return rc;
+        }
+        HIP_TRY(hipStreamWaitEvent(h->stream, ev_rc[q], 0));
         // the transposed flux of launch L+1 is fused into the update of L
         // (both touch lambda of L's targets); the segment's first launch
         // ends with it as a launch of its own
         for (int64_t L = nl - 1; L >= 0; --L) {
             const int kind = L % 2;
-            const smp::SubstepArgs<T> &f = launch_of(L);
-            const void *U0 = h->adj_rec[L];
-            const void *U = L + 1 < nl ? h->adj_rec[L + 1] : h->work;
+            const smp::SubstepArgs<T> &f = launch_of(q, L, nl);
+            const void *U0 = rec(q, L);
+            const void *U = L + 1 < nl ? rec(q, L + 1) : work_of[q];
             const int32_t *tl = kind ? list_even : list_odd, *sl = kind ? list_odd : list_even;
             const int ntl = kind ? nl_even : nl_odd, nsl = kind ? nl_odd : nl_even;
             HIP_TRY(smp::launch_adj_update<T>(adj_args(U, U0, tl, ntl, f.napp, f.fslot[0], f.fslot[1]),
@@ -1184,6 +1224,7 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
                                                               f.fslot[1]),
                                                      h->stream));
         }
+        HIP_TRY(hipEventRecord(ev_rev[q], h->stream));
     }
     HIP_TRY(hipEventRecord(e2, h->stream));
     HIP_TRY(hipEventSynchronize(e2));
@@ -1195,6 +1236,11 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipEventDestroy(e2);
+    (void)hipEventDestroy(ev_fwd);
+    for (int q = 0; q < 2; ++q) {
+        (void)hipEventDestroy(ev_rc[q]);
+        (void)hipEventDestroy(ev_rev[q]);
+    }
     return SMP_OK;
 }
 
