@@ -233,6 +233,18 @@ def adjoint_leg(n=512, n_mom=15, steps=2000, every=50, fd_cells=8, seed=2027):
         f_ms, r_ms = s.adjoint_timing()
         owner = dict(prob, sigma_a=(lambda x, y: prob["_owner"](x, y).astype(np.float64)))
         ga, gs = s.cell_gradients(owner, n * n)
+        # algorithmic bytes of one fused update launch (targets of one
+        # parity, mean over the two): per target and point U0, lambda
+        # read and written, mu written; per source and point its state and
+        # its mu read once; per grid type of the targets and point the two
+        # FP64 gradients read and written (coefficient fields not counted)
+        par0 = sum(s.gkey[0])
+        parity = [(sum(g) - par0) % 2 for g in s.gkey]
+        n_p = [parity.count(0), parity.count(1)]
+        n_g = [len({g for g, q in zip(s.gkey, parity) if q == k}) for k in (0, 1)]
+        e = 4
+        upd_bytes = float(np.mean([n * n * (4 * e * n_p[k] + 2 * e * n_p[1 - k] + 32 * n_g[k])
+                                   for k in (0, 1)]))
     finally:
         s.close()
     log("adjoint: forward %.2f s, adjoint forward+reverse %.2f s" % (fwd_ms / 1e3, (f_ms + r_ms) / 1e3))
@@ -244,6 +256,8 @@ def adjoint_leg(n=512, n_mom=15, steps=2000, every=50, fd_cells=8, seed=2027):
         "cost_vs_forward": (f_ms + r_ms) / fwd_ms,
         "checkpoint_every": every,
         "J": J32,
+        "update_alg_bytes_per_launch": upd_bytes,
+        "launches_per_step": {"forward": 2, "recompute": 2, "adjoint_update": 2},
     }
     if fd_cells > 0:
         rng = np.random.default_rng(seed)

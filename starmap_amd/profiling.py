@@ -6,11 +6,12 @@
     between each counter and the bytes a kernel of known size moves.
 
 ``python -m starmap_amd.profiling summarize KERNEL_DIR FETCH_DIR WRITE_DIR
-COPYF_DIR COPYW_DIR``
+COPYF_DIR COPYW_DIR [KERNEL]``
     reads the rocprofv3 CSV outputs (kernel-trace stats, the FETCH_SIZE and
-    WRITE_SIZE counter runs of the substep kernel and of the copy) and prints
-    a JSON summary: mean launch time of the substep kernel, calibrated HBM
-    bytes read and written per launch.
+    WRITE_SIZE counter runs of a kernel and of the copy) and prints a JSON
+    summary: mean launch time of the kernels whose name contains KERNEL
+    (default ``substep_kernel``), calibrated HBM bytes read and written per
+    launch.
 """
 import csv
 import glob
@@ -46,18 +47,19 @@ def _counter_mean(d, name, kernel_sub, skip_first=0):
     return sum(vals) / len(vals) if vals else None, len(vals)
 
 
-def summarize(kdir, fdir, wdir, cfdir, cwdir):
+def summarize(kdir, fdir, wdir, cfdir, cwdir, kernel="substep_kernel"):
     stats = {}
     for r in _rows(kdir, "*kernel_stats.csv"):
         stats[r["Name"]] = r
-    sub = [r for n, r in stats.items() if "substep_kernel" in n]
+    sub = [r for n, r in stats.items() if kernel in n]
     launches = sum(int(r["Calls"]) for r in sub)
     total_ns = sum(float(r["TotalDurationNs"]) for r in sub)
-    fetch_kib, nf = _counter_mean(fdir, "FETCH_SIZE", "substep_kernel")
-    write_kib, nw = _counter_mean(wdir, "WRITE_SIZE", "substep_kernel")
+    fetch_kib, nf = _counter_mean(fdir, "FETCH_SIZE", kernel)
+    write_kib, nw = _counter_mean(wdir, "WRITE_SIZE", kernel)
     cf, _ = _counter_mean(cfdir, "FETCH_SIZE", "copyBuffer")
     cw, _ = _counter_mean(cwdir, "WRITE_SIZE", "copyBuffer")
     out = {
+        "kernel": kernel,
         "substep_launches": launches,
         "substep_mean_ms": total_ns / launches / 1e6 if launches else None,
         "fetch_kib_per_launch": fetch_kib,
@@ -82,7 +84,7 @@ def summarize(kdir, fdir, wdir, cfdir, cwdir):
 if __name__ == "__main__":
     if len(sys.argv) >= 2 and sys.argv[1] == "copycal":
         copycal()
-    elif len(sys.argv) == 7 and sys.argv[1] == "summarize":
+    elif len(sys.argv) in (7, 8) and sys.argv[1] == "summarize":
         summarize(*sys.argv[2:])
     else:
         print(__doc__)
