@@ -234,7 +234,10 @@ int smp_adjoint_seed(smp_problem *h, int32_t j, const double *src, int64_t count
  * over the steps, dJ/dsigma_a and dJ/dsigma_s0 at every point of every
  * staggered grid type (sigma_t = sigma_a + sigma_s0 - sigma_sm for j > 1,
  * sigma_a for j = 1, starmap_solver.m:214-252).  Filters must be off or at
- * 'substep' position and constant in space; sources must be fields. */
+ * 'substep' position and constant in space; sources must be fields; each
+ * component slab must be below 2 GiB.  The recomputation runs on a second
+ * stream of the handle (default priority); the call returns after both
+ * streams are done. */
 int smp_adjoint(smp_problem *h, double dt, int64_t n_steps, int32_t checkpoint_every);
 
 /* Component j of the adjoint state (what = 0), or dJ/dsigma_a (what = 1) or

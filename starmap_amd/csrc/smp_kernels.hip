@@ -59,10 +59,10 @@ __device__ __forceinline__ SMP_GLOBAL P *glb(P *p) {
     return (SMP_GLOBAL P *)p;
 }
 
-// One workgroup = one spatial tile of up to 256 cells.  Waves 0-3 own one
-// cell per thread and update the targets; waves 4-5 copy: source tiles and the
-// group records go from HBM straight into LDS (global_load_lds, no
-// registers).  While the cell waves update the targets of group g from LDS,
+// One workgroup = one spatial tile of up to 256 cells.  The first CW waves
+// (cell waves) own one cell per thread and update the targets; the last two
+// waves copy: source tiles and the group records go from HBM straight into
+// LDS (global_load_lds, no registers).  While the cell waves update the targets of group g from LDS,
 // the copy wave brings in the source tiles of group g+1 and the records of
 // group g+2, and the cell waves load the target values of group g+1; one
 // barrier per group.  Only the copy waves have copies in flight, so the cell
