@@ -249,6 +249,22 @@ int smp_adjoint_get(smp_problem *h, int32_t what, int32_t j, double *dst, int64_
  * (HIP events on the handle's stream). */
 int smp_adjoint_timing(const smp_problem *h, double *forward_ms, double *reverse_ms);
 
+/* Moment norms of the current state, per component j in the reference's
+ * output layout: sumsq[j] = sum of U_j^2 over its grid points, diffsq[3*j+e]
+ * = sum of squared differences of neighbours along output dimension e+1
+ * (MATLAB diff(U,[],e+1)); the terms of the moment-order norms of
+ * starmap_2d_ex_lattice_filter_convergence.m:66-76 (grouping by moment order
+ * and the square roots stay with the caller).  diffsq has 3*n_sys entries,
+ * zero for dimensions the problem does not have. */
+int smp_moment_norms(smp_problem *h, double *sumsq, double *diffsq);
+
+/* Every component linearly interpolated at one point on its own staggered
+ * grid, as the interp2 loop of starmap_2d_ex_linesource_filter_reconstruction.m
+ * :118-121 (NaN outside the grid).  lo and point have 3 entries in the
+ * solver's axis order of smp_desc.n (lo: the lower domain corner
+ * par.ax(1:2:end)); out has n_sys entries. */
+int smp_probe(smp_problem *h, const double *lo, const double *point, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,7 +27,7 @@ EXPORTS = ["smp_last_error", "smp_version", "smp_create", "smp_destroy", "smp_gr
            "smp_set_schedule", "smp_get_schedule", "smp_checkpoint_save", "smp_checkpoint_load",
            "smp_checkpoint_free", "smp_batch_create", "smp_batch_destroy", "smp_batch_run",
            "smp_batch_timing", "smp_adjoint_reset", "smp_adjoint_seed", "smp_adjoint",
-           "smp_adjoint_get", "smp_adjoint_timing"]
+           "smp_adjoint_get", "smp_adjoint_timing", "smp_moment_norms", "smp_probe"]
 
 
 class SmpError(RuntimeError):
@@ -101,6 +101,8 @@ def lib():
         "smp_adjoint": (ctypes.c_int, [vp, dbl, i64, i32]),
         "smp_adjoint_get": (ctypes.c_int, [vp, i32, i32, vp, i64]),
         "smp_adjoint_timing": (ctypes.c_int, [vp, vp, vp]),
+        "smp_moment_norms": (ctypes.c_int, [vp, vp, vp]),
+        "smp_probe": (ctypes.c_int, [vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

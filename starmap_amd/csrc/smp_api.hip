@@ -101,6 +101,7 @@ struct smp_problem {
     void *interp_buf = nullptr;
     int64_t interp_cap = 0;
     void *red_buf = nullptr;
+    void *norm_buf = nullptr;   // partials and results of smp_moment_norms
     void *sink = nullptr;         // 512 elements: stores of lanes without a target cell
     std::vector<hipEvent_t> events;
     std::vector<void *> ckpt;     // device copies of the whole state
@@ -1607,6 +1608,123 @@ int smp_observables(smp_problem *h, double *sums, double *sq, int32_t *nonfinite
         if (r[2] != 0) bad = 1;
     }
     if (nonfinite) *nonfinite = bad;
+    return SMP_OK;
+}
+
+int smp_moment_norms(smp_problem *h, double *sumsq, double *diffsq) {
+    if (!h || !sumsq || !diffsq) return fail(SMP_E_INVALID, "null argument");
+    REQUIRE_DEVICE(h);
+    HIP_TRY(hipSetDevice(h->device));
+    const int blocks = 1024;
+    if (!h->norm_buf) {
+        int rc = h->dalloc(&h->norm_buf, sizeof(double) * (4 * (size_t)blocks + 4 * (size_t)h->geo.n_sys));
+        if (rc) return rc;
+    }
+    double *part = (double *)h->norm_buf;
+    double *res = part + 4 * blocks;
+    std::vector<double> r(4 * (size_t)h->geo.n_sys);
+    for (int j = 0; j < h->geo.n_sys; ++j) {
+        smp::NormGeom g;
+        std::memset(&g, 0, sizeof(g));
+        for (int e = 0; e < 3; ++e) {
+            if (e < h->geo.ndim) {
+                const int ax = h->geo.axis[e];
+                g.ext[e] = h->geo.ext_of(j, ax);
+                g.rot[e] = h->geo.gt[3 * j + ax] == 2 && h->geo.bc[ax] == 0;
+            } else {
+                g.ext[e] = 1;
+            }
+        }
+        g.pitch0 = h->pitch0;
+        g.ext1 = h->E[1];
+        if (h->precision == SMP_FP64)
+            HIP_TRY(smp::launch_norm<double>((const double *)h->U + (size_t)j * h->slab, g, part, blocks,
+                                             res + 4 * j, h->stream));
+        else
+            HIP_TRY(smp::launch_norm<float>((const float *)h->U + (size_t)j * h->slab, g, part, blocks,
+                                            res + 4 * j, h->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(r.data(), res, sizeof(double) * r.size(), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (int j = 0; j < h->geo.n_sys; ++j) {
+        sumsq[j] = r[4 * j];
+        for (int e = 0; e < 3; ++e) diffsq[3 * j + e] = r[4 * j + 1 + e];
+    }
+    return SMP_OK;
+}
+
+int smp_probe(smp_problem *h, const double *lo, const double *point, double *out) {
+    if (!h || !lo || !point || !out) return fail(SMP_E_INVALID, "null argument");
+    REQUIRE_DEVICE(h);
+    HIP_TRY(hipSetDevice(h->device));
+    const smp::Geometry &g = h->geo;
+    const int64_t stride[3] = {1, h->pitch0, (int64_t)h->pitch0 * h->E[1]};
+    std::vector<smp::ProbeComp> pc(g.n_sys);
+    for (int j = 0; j < g.n_sys; ++j) {
+        smp::ProbeComp &c = pc[j];
+        std::memset(&c, 0, sizeof(c));
+        // per kernel axis: the two stored indices and weights of the point
+        int64_t s0[3] = {0, 0, 0}, s1[3] = {0, 0, 0};
+        double t[3] = {0, 0, 0};
+        for (int e = 0; e < g.ndim; ++e) {
+            const int ax = g.axis[e];
+            const bool face = g.gt[3 * j + ax] == 2;
+            const int n = g.n[ax], bc = g.bc[ax];
+            const double hh = g.h[ax];
+            // the reference's grid vector of this grid type (starmap_solver.m:113-118)
+            const double first = lo[ax] + (face ? (bc ? 0.0 : hh) : hh / 2);
+            const int len = face ? n + bc : n;
+            double f = (point[ax] - first) / hh;
+            // the grid vector is first + k*h here and MATLAB's colon there:
+            // points within 1e-9 cells of either end are on it
+            if (f < 0 && f > -1e-9) f = 0;
+            if (f > len - 1 && f < len - 1 + 1e-9) f = len - 1;
+            if (!(f >= 0) || f > len - 1) {
+                c.nan = 1;
+                break;
+            }
+            int f0 = (int)std::floor(f);
+            if (f0 > len - 2) f0 = len > 1 ? len - 2 : 0;
+            t[e] = len > 1 ? f - f0 : 0.0;
+            const int f1 = len > 1 ? f0 + 1 : f0;
+            auto stored = [&](int fi) { return face && bc == 0 ? (fi + 1) % n : fi; };
+            s0[e] = stored(f0);
+            s1[e] = stored(f1);
+        }
+        if (c.nan) continue;
+        const int corners = 1 << g.ndim;
+        for (int k = 0; k < corners; ++k) {
+            double w = 1;
+            int64_t off = 0;
+            for (int e = 0; e < g.ndim; ++e) {
+                const bool hi = (k >> e) & 1;
+                w *= hi ? t[e] : 1 - t[e];
+                off += (hi ? s1[e] : s0[e]) * stride[e];
+            }
+            c.off[k] = off;
+            c.w[k] = w;
+        }
+        c.n = corners;
+    }
+    void *dpc = nullptr, *dout = nullptr;
+    int rc = h->dalloc(&dpc, sizeof(smp::ProbeComp) * pc.size());
+    if (!rc) rc = h->dalloc(&dout, sizeof(double) * pc.size());
+    if (rc) {
+        h->dfree(dpc);
+        return rc;
+    }
+    HIP_TRY(hipMemcpyAsync(dpc, pc.data(), sizeof(smp::ProbeComp) * pc.size(), hipMemcpyHostToDevice,
+                           h->stream));
+    if (h->precision == SMP_FP64)
+        HIP_TRY(smp::launch_probe<double>((const double *)h->U, h->slab, (const smp::ProbeComp *)dpc,
+                                          g.n_sys, (double *)dout, h->stream));
+    else
+        HIP_TRY(smp::launch_probe<float>((const float *)h->U, h->slab, (const smp::ProbeComp *)dpc,
+                                         g.n_sys, (double *)dout, h->stream));
+    HIP_TRY(hipMemcpyAsync(out, dout, sizeof(double) * pc.size(), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->dfree(dpc);
+    h->dfree(dout);
     return SMP_OK;
 }
 

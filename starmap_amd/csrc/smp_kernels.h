@@ -113,6 +113,32 @@ hipError_t launch_reduce(const T *U, const ReduceGeom &g, double *part, int bloc
 
 hipError_t launch_reduce_final(const double *part, int blocks, double *out3, hipStream_t s);
 
+// Moment norms of one component: sum of squares and, per kernel axis, of
+// squared differences of neighbours in the reference's output order
+// (rot[e]: periodic face axis, stored index (f+1) mod n).
+struct NormGeom {
+    int32_t ext[3];
+    int32_t pitch0, ext1;
+    int32_t rot[3];
+};
+
+// out4 = {sum u^2, sum d0^2, sum d1^2, sum d2^2}; part holds 4*blocks doubles
+template <typename T>
+hipError_t launch_norm(const T *U, const NormGeom &g, double *part, int blocks, double *out4,
+                       hipStream_t s);
+
+// Linear interpolation of one component at a point: up to 8 slab offsets
+// and weights (nan: the point is outside the component's grid).
+struct ProbeComp {
+    int64_t off[8];
+    double w[8];
+    int32_t n, nan;
+};
+
+template <typename T>
+hipError_t launch_probe(const T *U, int64_t slab, const ProbeComp *pc, int n, double *out,
+                        hipStream_t s);
+
 }  // namespace smp
 
 namespace smp {
