@@ -4,9 +4,9 @@ TEST INFRASTRUCTURE ONLY.  Imported by ``tests/``, ``__graft_entry__.smoke()``
 and the CPU-baseline leg of ``bench.py`` -- never by the product path.
 
 * ``starmap_init(prob)``   restates starmap_init.m:30-231
-* ``starmap_solver(par)``  restates starmap_solver.m:58-500 for the hyperbolic
-  time step (``timestep == 0``); the parabolic step-size search of
-  :171-204/:253-265 is not restated and raises.
+* ``starmap_solver(par)``  restates starmap_solver.m:58-500, with the
+  parabolic step-size search of :171-204/:253-265 (``growth_factor``
+  :506-514, ``timestep`` 1 or 2).
 
 The host bookkeeping (flags, coupling lists, grid-type classes, grids, dt,
 materials, decay terms, sources, filters, output interpolation) is numpy,
@@ -271,15 +271,62 @@ def _keep(a):
     return np.asfortranarray(np.asarray(a, dtype=np.float64))
 
 
+def growth_factor(M, c_even, c_odd, decay, dt, timestep):
+    """starmap_solver.m:506-514: largest |eig| of G_o*G_e*G_e*G_o with
+    G = diag(dt*expm1div(-decay*dt/2))*(M+diag(-decay/2)), G_e (G_o) = G with
+    the odd (even) rows zeroed plus I; times exp(decay(1)*dt) for
+    timestep 2.  c_even, c_odd are 0-based index lists."""
+    decay = np.asarray(decay, dtype=np.float64).ravel()
+    G = np.diag(dt * expm1div(-decay * dt / 2)) @ (M + np.diag(-decay / 2))
+    I = np.eye(decay.size)
+    G_e = G.copy()
+    for r in c_odd:
+        G_e[r, :] = 0
+    G_e = G_e + I
+    G_o = G.copy()
+    for r in c_even:
+        G_o[r, :] = 0
+    G_o = G_o + I
+    G = G_o @ G_e @ G_e @ G_o
+    la = np.max(np.abs(np.linalg.eigvals(G)))
+    if timestep == 2:
+        la = la * np.exp(decay[0] * dt)
+    return float(la)
+
+
+def _stable_dt(M, c_even, c_odd, decay, dt, tleft, timestep):
+    """starmap_solver.m:181-203: bracket (doubling / halving) and geometric
+    bisection, in the reference's 1-based index arithmetic.  Returns
+    (dt_bracket(1), la0)."""
+    la = growth_factor(M, c_even, c_odd, decay, dt, timestep)
+    dt_bracket = np.array([1.0, 1.0]) * dt
+    la_bracket = np.array([1.0, 1.0]) * la
+    la0 = 1 + 1e-8
+    while not np.diff((la_bracket < la0).astype(int))[0] and \
+            (dt_bracket[0] <= tleft or la_bracket[0] > la0):
+        ind = int(la_bracket[0] < la0) + 1                   # 1: dt(1) unstable, 2: stable
+        dt_bracket = dt_bracket[ind - 1] * np.array([1.0, 2.0]) * 2.0 ** (ind - 2)
+        la_bracket = la_bracket[[1, 0]]
+        la_bracket[ind - 1] = growth_factor(M, c_even, c_odd, decay, dt_bracket[ind - 1], timestep)
+        if np.min(la_bracket) < 1:
+            la0 = 1
+    while dt_bracket[0] / dt_bracket[1] < 0.98 and dt_bracket[0] <= tleft:
+        dt_mid = np.sqrt(np.prod(dt_bracket))
+        la_mid = growth_factor(M, c_even, c_odd, decay, dt_mid, timestep)
+        ind = 1 + int(la_mid >= la0)
+        dt_bracket[ind - 1] = dt_mid
+        la_bracket[ind - 1] = la_mid
+    return float(dt_bracket[0]), la0
+
+
 def starmap_solver(par, verbose=True, return_state=False):
-    """Restatement of starmap_solver.m:58-500 (hyperbolic time step only).
+    """Restatement of starmap_solver.m:58-500.
 
     Returns the reference's output struct array as a list of dicts (only if
     the reference would return it, i.e. always here), plus bookkeeping in
     ``par['_oracle']`` (step count, dt, final t)."""
     par = dict(par)
-    if par.get("timestep", 0) > 0:
-        raise NotImplementedError("parabolic time step (starmap_solver.m:171-204) not restated")
+    timestep = int(par.get("timestep", 0))
     Mx, My, Mz = par["Mx"], par["My"], par["Mz"]
     n_sys = Mz.shape[0]
     # flags, :58-71
@@ -313,6 +360,8 @@ def starmap_solver(par, verbose=True, return_state=False):
         dt_hyp = dt_hyp / 3
     elif nn[1] != 1:
         dt_hyp = dt_hyp / 2
+    M = -1j * (np.asarray(Mx, dtype=np.float64) / h[0] + np.asarray(My, dtype=np.float64) / h[1] +
+               np.asarray(Mz, dtype=np.float64) / h[2])            # :108
     time_dependent = [nargin(par["sigma_a"]) > 3, nargin(par["sigma_s0"]) > 3,
                       nargin(par["sigma_sm"]) > 4]
     # grids, :113-126
@@ -377,7 +426,15 @@ def starmap_solver(par, verbose=True, return_state=False):
         for k in range(int(t > 0), 2):
             if not any(time_dependent) and t > 0:
                 break
-            dt_bracket = dt_hyp
+            flag_check_stab = k > 0 and timestep > 0
+            if flag_check_stab:
+                # :173-203, smallest decay rates of the current parameters
+                dt = max(dt, dt_hyp)
+                decay = [np.min(sA[(1, 1, 1)])] + [np.min(sT[gkey[j] + (int(mom_order[j]),)])
+                                                   for j in range(1, n_sys)]
+                dt_bracket, la0 = _stable_dt(M, even, odd, decay, dt, tleft, timestep)
+            else:
+                dt_bracket = dt_hyp
             if k > 0:
                 dt = min(dt_bracket, tleft) * par["cn"]
                 sA0, sT0 = sA, sT
@@ -401,6 +458,14 @@ def starmap_solver(par, verbose=True, return_state=False):
             if k > 0:
                 sA1 = (sA0[(1, 1, 1)] + sA[(1, 1, 1)]) / 2
                 sT1 = {s: (sT0[s] + sT[s]) / 2 for s in sg}
+            if flag_check_stab:
+                # :253-265: fall back to the parameters at t if the averaged
+                # ones are unstable with dt
+                decay = [np.min(sA1)] + [np.min(sT1[gkey[j] + (int(mom_order[j]),)])
+                                         for j in range(1, n_sys)]
+                if not growth_factor(M, even, odd, decay, dt, timestep) < la0:
+                    sA1 = sA0[(1, 1, 1)]
+                    sT1 = sT0
         # decay terms, :268-274
         if t == 0 or any(time_dependent):
             EA = expm1div(-np.asarray(sA1, dtype=np.float64) * dt / 2)

@@ -263,6 +263,37 @@ def batch_sweep(n_instances=64, seed=2025, tfinal=1.0, side_div=1, max_order=Non
     return out
 
 
+def asymptotic_preserving_2d(n=150, n_mom=1, ep=1e-3, timestep=1, sigma_a=0.0, n_plot=21,
+                             time_dependent=False, half=0.6):
+    """starmap_2d_ex_asymptotic_preserving.m: 2D P_N on [-half,half]^2 with
+    extrapolation BC, isotropic scattering 1/ep (times 1 + 0.5 sin(t) if
+    time_dependent), smoothed Dirac initial zeroth moment (pseudo-time
+    3.2e-4), output times linspace(0, 0.005, n_plot)/ep; timestep 0
+    (hyperbolic), 1 or 2 (parabolic)."""
+    def ic(x, y, k):
+        t0 = 3.2e-4
+        return 1 / (4 * np.pi * t0) * np.exp(-(x ** 2 + y ** 2) / (4 * t0)) * (k == 1)
+
+    prob = {
+        "name": "Asymptotic Preserving Test",
+        "closure": "P",
+        "n_mom": n_mom,
+        "sigma_a": (lambda x, y: sigma_a + 0 * x),
+        "ic": ic,
+        "ax": [-half, half, -half, half],
+        "n": [n, n],
+        "bc": [1, 1],
+        "t_plot": list(np.linspace(0, 0.005, n_plot) / ep),
+        "timestep": timestep,
+        "output": None,
+    }
+    if time_dependent:
+        prob["sigma_s0"] = (lambda x, y, t: (1 + 0.5 * np.sin(t)) / ep + 0 * x)
+    else:
+        prob["sigma_s0"] = (lambda x, y: 1 / ep + 0 * x)
+    return prob
+
+
 def gaussian_random_field(n, corr=0.2, std=0.5, seed=2026, length=2.0):
     """Periodic Gaussian random field on the n x n cell grid of a square of
     side `length`: white noise from numpy default_rng(seed) filtered in

@@ -18,6 +18,7 @@ import time
 import numpy as np
 
 from . import _lib
+from . import timestep as _ts
 from .init import capargs, nargin, zero
 
 
@@ -354,11 +355,17 @@ class HipSolver:
 
     # -- the time loop ----------------------------------------------------
     def run(self, verbose=True):
-        """The time loop of starmap_solver.m:162-475 (hyperbolic time step).
-        Returns the reference's output struct array as a list of dicts."""
+        """The time loop of starmap_solver.m:162-475, hyperbolic or (par.timestep
+        1 or 2) parabolic time step.  Returns the reference's output struct
+        array as a list of dicts."""
         par = self.par
-        if par.get("timestep", 0) > 0:
-            raise NotImplementedError("parabolic time step selection (timestep > 0) is not supported")
+        timestep = int(par.get("timestep", 0))
+        if timestep > 0:
+            Mx, My, Mz = (np.asarray(par[k], dtype=np.float64) for k in ("Mx", "My", "Mz"))
+            M_sym = _ts.symbol(Mx, My, Mz, self.h)
+            even_g = {(1, 1, 1), (2, 2, 1), (2, 1, 2), (1, 2, 2)}
+            c_even = [j for j in range(self.n_sys) if self.gkey[j] in even_g]
+            c_odd = [j for j in range(self.n_sys) if self.gkey[j] not in even_g]
         L = _lib.lib()
         mom_output = [int(j) - 1 for j in np.atleast_1d(par["mom_output"])]
         td_mat = any(self.time_dependent)
@@ -383,7 +390,37 @@ class HipSolver:
         while t < tfinal:
             c0 = time.perf_counter()
             tleft = (tfinal - t) / cn
-            if t == 0 or td_mat:
+            if timestep > 0 and (t == 0 or td_mat):
+                flush()
+                # materials and dt, :168-266 with the stability search
+                for k in range(int(t > 0), 2):
+                    check = k > 0
+                    if check:
+                        dt = max(dt, self.dt_hyp)
+                        decay = _ts.decay_rates(sA[(1, 1, 1)], sT, self.gkey, self.mom_order)
+                        dt_b, la0 = _ts.select_dt(M_sym, c_even, c_odd, decay, dt, tleft, timestep)
+                    else:
+                        dt_b = self.dt_hyp
+                    if k > 0:
+                        dt = min(dt_b, tleft) * cn
+                        sA0, sT0 = sA, sT
+                        if not td_mat:
+                            sA1, sT1 = sA[(1, 1, 1)], sT
+                            break
+                    ev = [self.time_dependent[0] or dt == 0, self.time_dependent[1] or dt == 0,
+                          self.time_dependent[2] or (dt == 0 and self.flag_ani)]
+                    if any(ev):
+                        sA, sT = self.evaluate_materials(t + dt, ev)
+                    if k > 0:
+                        sA1 = (np.asarray(sA0[(1, 1, 1)]) + np.asarray(sA[(1, 1, 1)])) / 2
+                        sT1 = {q: (np.asarray(sT0[q]) + np.asarray(sT[q])) / 2 for q in self.sg}
+                    if check:
+                        # :253-265: parameters at t if the averaged ones are unstable
+                        d1 = _ts.decay_rates(sA1, sT1, self.gkey, self.mom_order)
+                        if not _ts.growth_factor(M_sym, c_even, c_odd, d1, dt, timestep) < la0:
+                            sA1, sT1 = sA0[(1, 1, 1)], sT0
+                self.upload_materials(sA1, sT1)
+            elif t == 0 or td_mat:
                 flush()
                 # materials and dt, :168-266 (hyperbolic branch)
                 if t == 0:

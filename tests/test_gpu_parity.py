@@ -61,6 +61,12 @@ CASES = {
     # SP_N with a time-dependent source
     "boxes_sp5": lambda: P.boxes_2d(n=40, n_mom=5, closure="SP", tfinal=0.3),
     "boxes_p3": lambda: P.boxes_2d(n=36, n_mom=3, closure="P", tfinal=0.2),
+    # parabolic time step (starmap_solver.m:171-204, :253-265)
+    "ap_p1_parabolic": lambda: P.asymptotic_preserving_2d(n=40, n_plot=3),
+    "ap_p3_parabolic_relative": lambda: P.asymptotic_preserving_2d(n=30, n_mom=3, timestep=2,
+                                                                   sigma_a=0.5, n_plot=3),
+    "ap_p1_parabolic_timedep": lambda: P.asymptotic_preserving_2d(n=30, n_plot=3,
+                                                                  time_dependent=True),
 }
 
 
