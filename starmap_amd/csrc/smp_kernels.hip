@@ -54,6 +54,9 @@ typedef __attribute__((address_space(3))) void *lds_ptr_t;
 // these casts state that they address global memory.
 #define SMP_GLOBAL __attribute__((address_space(1)))
 #define SMP_CONST __attribute__((address_space(4)))
+#ifndef SMP_SCALAR_RECORDS
+#define SMP_SCALAR_RECORDS 1
+#endif
 template <typename P>
 __device__ __forceinline__ SMP_GLOBAL P *glb(P *p) {
     return (SMP_GLOBAL P *)p;
@@ -236,8 +239,24 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     SMP_GLOBAL const char *ug_ld = glb(reinterpret_cast<const char *>(a.U)) + (in_store ? gidx : 0) * sizeof(T);
     SMP_GLOBAL char *ug_st = glb(reinterpret_cast<char *>(a.U)) + gidx * sizeof(T);
     SMP_GLOBAL T *sink = glb(a.sink);
-    auto load_targets = [&](int buf, T *u) {
-        const TgtRec *r = recs(buf) + half * TPT;
+    // The cell waves' records of group gq: with SMP_SCALAR_RECORDS, read
+    // with scalar loads from the plan in global memory (every workgroup
+    // reads the same records, which stay in the scalar cache and L2);
+    // otherwise from the group's blob in the LDS ring.
+#if SMP_SCALAR_RECORDS
+    const int half_u = __builtin_amdgcn_readfirstlane(half);
+    auto rec_of = [&](int gq, int buf) {
+        (void)buf;
+        return (const SMP_CONST TgtRec *)(a.blobs + (int64_t)gq * BLOB_SZ + BLOB_REC_OFF) + half_u * TPT;
+    };
+#else
+    auto rec_of = [&](int gq, int buf) {
+        (void)gq;
+        return recs(buf) + half * TPT;
+    };
+#endif
+    auto load_targets = [&](int gq, int buf, T *u) {
+        const auto r = rec_of(gq, buf);
 #pragma unroll
         for (int i = 0; i < TPT; ++i) u[i] = *reinterpret_cast<SMP_GLOBAL const T *>(ug_ld + r[i].uoff);
     };
@@ -260,28 +279,26 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     // Every group holds GMAX target records; unused ones are zero records
     // (extent 0, terms in the zero tile), so the loop has no branches and the
     // compiler can overlap one target's LDS reads with another's arithmetic.
-    auto update = [&](int buf, const T *u_in) {
-        const TgtRec *rb = recs(buf) + half * TPT;
+    auto update = [&](int gq, int buf, const T *u_in) {
+        const auto rb = rec_of(gq, buf);
 #pragma unroll
         for (int i = 0; i < TPT; ++i) {
-            const TgtRec &r = rb[i];
-            const int4 h0 = reinterpret_cast<const int4 *>(&r)[0];   // comp filt sig_off E_off
-            const int4 h1 = reinterpret_cast<const int4 *>(&r)[1];   // Q_off ext0 ext1 ext2
-            const bool act = has_cell & (g0 < h1.y) & (g1 < h1.z) & (g2 < h1.w);
+            const auto &r = rb[i];
+            const bool act = has_cell & (g0 < r.ext[0]) & (g1 < r.ext[1]) & (g2 < r.ext[2]);
             // W = -sumcell(...): the coefficients carry -M(j,k)/h; padding
             // terms add coef 0 times a difference within the zero tile
             A w = 0;
 #pragma unroll
             for (int q = 0; q < NT; ++q) {
-                const Term tm = r.t[q];
-                const unsigned char *lo = smem_raw + (tm.off + lb);
-                const A c = sizeof(T) == 4 ? (A)tm.coef32[0] : (A)tm.coef;
-                w += c * ((A)*reinterpret_cast<const T *>(lo + tm.stride) -
+                const int toff = r.t[q].off, tstride = r.t[q].stride;
+                const unsigned char *lo = smem_raw + (toff + lb);
+                const A c = sizeof(T) == 4 ? (A)r.t[q].coef32[0] : (A)r.t[q].coef;
+                w += c * ((A)*reinterpret_cast<const T *>(lo + tstride) -
                           (A)*reinterpret_cast<const T *>(lo));
             }
-            const A sig = coef(h0.z, r.sig_p, r.sig_s);
-            const A E = coef(h0.w, r.E_p, r.E_s);
-            const A wq = w + coef(h1.x, r.Q_p, 0.0);
+            const A sig = coef(r.sig_off, r.sig_p, r.sig_s);
+            const A E = coef(r.E_off, r.E_p, r.E_s);
+            const A wq = w + coef(r.Q_off, r.Q_p, 0.0);
             A u = (A)u_in[i];
 #pragma unroll
             for (int app = 0; app < NAPP; ++app) {
@@ -289,7 +306,7 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
                 u = u + (hdt * (wq - sig * u)) * E;
                 const bool s1 = (app == 0 ? fslot0 : fslot1) != 0;
                 A f = (A)(s1 ? r.F_s[1] : r.F_s[0]);   // 1 for targets without a filter
-                if (GEN && h0.y) {
+                if (GEN && r.filt) {
                     const void *fp = s1 ? r.F[1] : r.F[0];
                     if (fp && act) f = (A)glb(reinterpret_cast<const T *>(fp))[gidx];
                 }
@@ -342,15 +359,15 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     }
     for (int e = tid; e < S; e += NCT) lds[e] = (T)0;   // slot 0: the zero tile
     __syncthreads();
-    if (ng > 0) load_targets(0, uA);
+    if (ng > 0) load_targets(0, 0, uA);
     __syncthreads();
     for (int gq = 0; gq < ng; gq += 2) {
-        update(gq % 3, uA);
-        if (gq + 1 < ng) load_targets((gq + 1) % 3, uB);
+        update(gq, gq % 3, uA);
+        if (gq + 1 < ng) load_targets(gq + 1, (gq + 1) % 3, uB);
         __syncthreads();
         if (gq + 1 >= ng) break;
-        update((gq + 1) % 3, uB);
-        if (gq + 2 < ng) load_targets((gq + 2) % 3, uA);
+        update(gq + 1, (gq + 1) % 3, uB);
+        if (gq + 2 < ng) load_targets(gq + 2, (gq + 2) % 3, uA);
         __syncthreads();
     }
 }
