@@ -10,7 +10,9 @@ substep position, dt = 0.5*dx.  Inputs are resident in HBM before timing.
 
 Prints ONE JSON line: value = FP64 moment-cell updates/s
 (mcu/s = n_sys * cells * steps / wall), the roofline of the substep kernel
-(algorithmic bytes per launch / mean launch time from HIP events on the
+(algorithmic bytes per launch -- B_alg bytes per moment-cell update, the
+compulsory traffic of the reference's three-substep splitting, times the
+updates per launch -- over the mean launch time from HIP events on the
 library's stream), the FP32-state variant with its error against the FP64
 state after the same steps (each on the schedule the autotuner picks), and a CPU baseline (the FP64 oracle on the host
 cores, a bounded sample of the same workload).  A second leg ("gauss3d")
@@ -105,22 +107,28 @@ def fp32_error(s64, s32):
 
 
 def kernel_roofline(s, steps, cells, wall, elem):
-    """Substep-kernel bytes per launch, mean launch time and achieved GB/s,
-    plus the throughput priced with B_alg."""
+    """Achieved GB/s of the substep kernel: B_alg bytes per moment-cell
+    update times the updates of the timed launches, over the kernel's own
+    time (HIP events on the library stream); the kernel's staged bytes
+    (each source component read once, each target read and written once
+    per launch) beside it."""
     tot_ms, sub_ms, launches, sub_bytes = s.timing()
     n_even = int(np.sum((s.gt - 1).sum(axis=1) % 2 == 0))
     n_odd = s.n_sys - n_even
     bpm = b_alg_per_mcu(n_even, n_odd, s.n_sys, elem)
     mcu = s.n_sys * cells * steps
-    ach = sub_bytes / launches / (sub_ms / 1e3 / launches) / 1e9
+    ach = bpm * mcu / (sub_ms / 1e3) / 1e9
+    kb = sub_bytes / (sub_ms / 1e3) / 1e9
     return {
         "value": mcu / wall,
         "ms_per_step": wall / steps * 1e3,
         "achieved_GBs": ach,
         "frac": ach / HBM_PEAK_GBS,
-        "bytes_per_launch": sub_bytes / launches,
+        "bytes_per_launch": bpm * mcu / launches,
         "mean_launch_ms": sub_ms / launches,
-        "b_alg_frac": bpm * mcu / wall / 1e9 / HBM_PEAK_GBS,
+        "kernel_bytes_per_launch": sub_bytes / launches,
+        "kernel_bytes_GBs": kb,
+        "kernel_bytes_frac": kb / HBM_PEAK_GBS,
     }
 
 
@@ -170,6 +178,11 @@ def batched_leg(fp64_plain_value, n_instances=64, check=True):
         b.step(ns, use_graph=True)
         tot_ms, ker_ms, launches, by, mcu, grid = b.timing()
         log("batched: %d launches in %.3f s" % (launches, tot_ms / 1e3))
+        b_alg_bytes = 0.0
+        for s_i, n_i in zip(b.solvers, ns):
+            ne = int(np.sum((s_i.gt - 1).sum(axis=1) % 2 == 0))
+            cells_i = int(np.prod([s_i.n[a] for a in range(3)]))
+            b_alg_bytes += b_alg_per_mcu(ne, s_i.n_sys - ne, s_i.n_sys, 4) * s_i.n_sys * cells_i * int(n_i)
         leg = {
             "workload": "%d 2D P_N problems, sides 64..512, odd N 3..39, FP32 state" % len(pars),
             "value": mcu / (tot_ms / 1e3),
@@ -180,8 +193,10 @@ def batched_leg(fp64_plain_value, n_instances=64, check=True):
             "launches": launches,
             "graph": True,
             "persistent_grid": grid,
-            "achieved_GBs": by / (tot_ms / 1e3) / 1e9,
-            "frac": by / (tot_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+            # B_alg (FP32) of every problem times its updates, over the run
+            "achieved_GBs": b_alg_bytes / (tot_ms / 1e3) / 1e9,
+            "frac": b_alg_bytes / (tot_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "kernel_bytes_GBs": by / (tot_ms / 1e3) / 1e9,
             "schedule": {"tile": list(tile), "blocks_per_cu": bpc, "cell_waves": cw,
                          "candidates_ms": {"%s/%d/%d" % c: v for c, v in times.items()}},
         }
@@ -366,12 +381,16 @@ def main():
     mcu = n_sys * cells * args.steps
     elem = 8 if main_key == "fp64" else 4
     bpm = b_alg_per_mcu(n_even, n_odd, n_sys, elem)
-    # the substep kernel's own algorithmic bytes per launch (each staged
-    # source component read once, each target read and written once), over
-    # its mean launch time from HIP events on the library's stream
-    bytes_per_launch = sub_bytes / launches
+    # algorithmic bytes per launch: B_alg per moment-cell update (the
+    # compulsory traffic of the reference's three-substep splitting) times
+    # the updates of the timed launches over their number; achieved = that
+    # over the mean launch time from HIP events on the library's stream
     mean_launch_s = sub_ms / 1e3 / launches
+    bytes_per_launch = bpm * mcu / launches
     achieved = bytes_per_launch / mean_launch_s / 1e9
+    # the bytes the fused launches stage (each source component read once,
+    # each target read and written once per launch)
+    kernel_bytes_per_launch = sub_bytes / launches
     # HBM bytes per launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE runs of
     # this workload, calibrated with a copy kernel (profiles/*_traffic.json)
     traffic = None
@@ -410,18 +429,21 @@ def main():
             "traffic": traffic,
             "kernel": "substep_kernel",
             "bytes_per_launch": bytes_per_launch,
+            "b_alg_per_mcu": bpm,
+            "mcu_per_launch": mcu / launches,
             "mean_launch_ms": mean_launch_s * 1e3,
             "launches": launches,
-            # the same throughput priced with the reference's three-substep
-            # splitting (B_alg bytes per moment-cell update)
-            "b_alg_per_mcu": bpm,
-            "b_alg_GBs": bpm * mcu / wall / 1e9,
-            "b_alg_frac": bpm * mcu / wall / 1e9 / HBM_PEAK_GBS,
+            "kernel_bytes_per_launch": kernel_bytes_per_launch,
+            "kernel_bytes_GBs": kernel_bytes_per_launch / mean_launch_s / 1e9,
+            "kernel_bytes_frac": kernel_bytes_per_launch / mean_launch_s / 1e9 / HBM_PEAK_GBS,
+            # the whole-step rate (wall clock, host gaps included) in B_alg terms
+            "b_alg_wall_GBs": bpm * mcu / wall / 1e9,
         },
     }
     if "fp32" in res and "fp64" in res:
         s32, wall32, (sub32, l32, by32) = res["fp32"]
-        ach32 = by32 / (sub32 / 1e3) / 1e9
+        bpm32 = b_alg_per_mcu(n_even, n_odd, n_sys, 4)
+        ach32 = bpm32 * n_sys * cells * args.steps / (sub32 / 1e3) / 1e9
         err = fp32_error(s64, s32)
         out["fp32"] = dict({
             "value": mcu / wall32,
