@@ -43,7 +43,7 @@ def _run(cmd):
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     hipcc = _hipcc()
-    objs = []
+    objs, cmds = [], []
     for src in sorted(os.listdir(CSRC)):
         path = os.path.join(CSRC, src)
         obj = os.path.join(BUILD, src + ".o")
@@ -55,8 +55,13 @@ def build(verbose=False):
             continue
         if verbose:
             print(" ".join(cmd))
-        _run(cmd)
+        cmds.append(cmd)
         objs.append(obj)
+    # the sources compile independently: one process each
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=min(len(cmds), os.cpu_count() or 1, 8)) as pool:
+        for f in [pool.submit(_run, c) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     cmd = [hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
     if verbose:
