@@ -1265,6 +1265,7 @@ struct smp_batch {
     int g_launches = -1, g_grid = 0, g_nt = 0, g_cw = 0;
     size_t g_lds = 0;
     bool g_gen = false;
+    std::vector<int> g_napp;   // applications of each captured launch
     std::vector<hipEvent_t> events;
     double last_ms = 0, last_kernel_ms = 0, last_bytes = 0;
     int64_t last_launches = 0, last_mcu = 0;
@@ -1336,46 +1337,60 @@ int batch_run(smp_batch *b, const double *dt, const int64_t *ns, int use_graph) 
             lds = std::max(lds, h->lds_bytes[kind]);
         }
     }
-    const int n_launch = (int)(2 * max_n + 1);
+    // Launch slots: substep position L of the lockstep, split by the number
+    // of half-step applications (the problems ending at L take one, the
+    // others two), so that each launch runs one compiled tile path.  The
+    // entries of a position are independent problems.
+    const int n_pos = (int)(2 * max_n + 1);
     std::vector<smp::BatchEntry<T>> ent;
-    std::vector<smp::BatchLaunch<T>> desc(n_launch);
-    std::vector<size_t> first(n_launch + 1, 0);
+    std::vector<smp::BatchLaunch<T>> desc;
+    std::vector<int> lnapp;
+    std::vector<size_t> first;
     double bytes = 0;
-    for (int L = 0; L < n_launch; ++L) {
-        first[L] = ent.size();
+    for (int L = 0; L < n_pos; ++L) {
         const int64_t k = L / 2;
-        for (int i = 0; i < n; ++i) {
-            if (ns[i] == 0) continue;
-            const smp::SubstepArgs<T> *a = nullptr;
-            if (L == 0)
-                a = &var[i].odd_first;
-            else if (L & 1)
-                a = k < ns[i] ? &var[i].even : nullptr;
-            else
-                a = k < ns[i] ? &var[i].odd_fused : (k == ns[i] ? &var[i].odd_last : nullptr);
-            if (!a) continue;
-            smp::BatchEntry<T> e;
-            std::memset(&e, 0, sizeof(e));
-            e.a = *a;
-            e.pad = i;
-            ent.push_back(e);
-            bytes += substep_bytes(b->hs[i], (L & 1) ? 1 : 0);
+        for (int napp = 2; napp >= 1; --napp) {
+            const size_t f = ent.size();
+            for (int i = 0; i < n; ++i) {
+                if (ns[i] == 0) continue;
+                const smp::SubstepArgs<T> *a = nullptr;
+                if (L == 0)
+                    a = &var[i].odd_first;
+                else if (L & 1)
+                    a = k < ns[i] ? &var[i].even : nullptr;
+                else
+                    a = k < ns[i] ? &var[i].odd_fused : (k == ns[i] ? &var[i].odd_last : nullptr);
+                if (!a || a->napp != napp) continue;
+                smp::BatchEntry<T> e;
+                std::memset(&e, 0, sizeof(e));
+                e.a = *a;
+                e.pad = i;
+                ent.push_back(e);
+                bytes += substep_bytes(b->hs[i], (L & 1) ? 1 : 0);
+            }
+            if (ent.size() == f) continue;
+            // heaviest tiles first (most groups per tile), so that the last
+            // units handed out are the cheap ones
+            std::stable_sort(ent.begin() + f, ent.end(),
+                             [](const smp::BatchEntry<T> &x, const smp::BatchEntry<T> &y) {
+                                 return x.a.ngroups > y.a.ngroups;
+                             });
+            int32_t u = 0;
+            for (size_t q = f; q < ent.size(); ++q) {
+                ent[q].unit0 = u;
+                u += ent[q].a.n_tiles;
+            }
+            smp::BatchLaunch<T> d;
+            std::memset(&d, 0, sizeof(d));
+            d.n_entries = (int32_t)(ent.size() - f);
+            d.n_units = u;
+            desc.push_back(d);
+            lnapp.push_back(napp);
+            first.push_back(f);
         }
-        // heaviest tiles first (most groups per tile), so that the last
-        // units handed out are the cheap ones
-        std::stable_sort(ent.begin() + first[L], ent.end(),
-                         [](const smp::BatchEntry<T> &x, const smp::BatchEntry<T> &y) {
-                             return x.a.ngroups > y.a.ngroups;
-                         });
-        int32_t u = 0;
-        for (size_t q = first[L]; q < ent.size(); ++q) {
-            ent[q].unit0 = u;
-            u += ent[q].a.n_tiles;
-        }
-        desc[L].n_entries = (int32_t)(ent.size() - first[L]);
-        desc[L].n_units = u;
     }
-    first[n_launch] = ent.size();
+    const int n_launch = (int)desc.size();
+    first.push_back(ent.size());
     // device tables (kept between calls: a captured graph refers to them)
     HIP_TRY(hipSetDevice(b->device));
     const size_t ebytes = sizeof(smp::BatchEntry<T>) * std::max<size_t>(1, ent.size());
@@ -1412,7 +1427,10 @@ int batch_run(smp_batch *b, const double *dt, const int64_t *ns, int use_graph) 
             return fail(SMP_E_INVALID, "batched handles must share cell_waves");
     int ncu = 0, per_cu = 0;
     HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device));
-    HIP_TRY(smp::batch_occupancy<T>(nt, gen, cw, lds, &per_cu));
+    int per_cu1 = 0;
+    HIP_TRY(smp::batch_occupancy<T>(nt, gen, cw, 2, lds, &per_cu));
+    HIP_TRY(smp::batch_occupancy<T>(nt, gen, cw, 1, lds, &per_cu1));
+    per_cu = std::min(per_cu, per_cu1);
     if (per_cu < 1) return fail(SMP_E_INVALID, "the batched kernel does not fit on a CU");
     const int grid = ncu * per_cu;
     b->grid = grid;
@@ -1422,7 +1440,7 @@ int batch_run(smp_batch *b, const double *dt, const int64_t *ns, int use_graph) 
         HIP_TRY(smp::launch_fill(b->counters, n_launch, 0, b->stream));
         for (int L = 0; L < n_launch; ++L) {
             if (timed) HIP_TRY(hipEventRecord(b->events[2 + 2 * L], b->stream));
-            HIP_TRY(smp::launch_batch<T>(dl + L, nt, gen, cw, grid, lds, b->stream));
+            HIP_TRY(smp::launch_batch<T>(dl + L, nt, gen, cw, lnapp[L], grid, lds, b->stream));
             if (timed) HIP_TRY(hipEventRecord(b->events[3 + 2 * L], b->stream));
         }
         return SMP_OK;
@@ -1435,7 +1453,8 @@ int batch_run(smp_batch *b, const double *dt, const int64_t *ns, int use_graph) 
     }
     if (use_graph) {
         const bool same = b->gexec && b->g_launches == n_launch && b->g_grid == grid &&
-                          b->g_lds == lds && b->g_nt == nt && b->g_gen == gen && b->g_cw == cw;
+                          b->g_lds == lds && b->g_nt == nt && b->g_gen == gen && b->g_cw == cw &&
+                          b->g_napp == lnapp;
         if (!same) {
             if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
             if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -1456,6 +1475,7 @@ int batch_run(smp_batch *b, const double *dt, const int64_t *ns, int use_graph) 
             b->g_nt = nt;
             b->g_gen = gen;
             b->g_cw = cw;
+            b->g_napp = lnapp;
         }
         HIP_TRY(hipEventRecord(b->events[0], b->stream));
         HIP_TRY(hipGraphLaunch(b->gexec, b->stream));

@@ -76,13 +76,14 @@ hipError_t launch_substep(const SubstepArgs<T> &a, int nt, bool gen, int cw, siz
 
 // Persistent batched substep launch over `grid` workgroups; bl is a device
 // pointer.  nt / gen as for launch_substep, over all entries.
+// napp: the half-step applications of every entry of the launch (1 or 2)
 template <typename T>
-hipError_t launch_batch(const BatchLaunch<T> *bl, int nt, bool gen, int cw, int grid,
+hipError_t launch_batch(const BatchLaunch<T> *bl, int nt, bool gen, int cw, int napp, int grid,
                         size_t lds_bytes, hipStream_t s);
 
 // workgroups of the batched kernel resident per CU with lds_bytes of LDS
 template <typename T>
-hipError_t batch_occupancy(int nt, bool gen, int cw, size_t lds_bytes, int *blocks);
+hipError_t batch_occupancy(int nt, bool gen, int cw, int napp, size_t lds_bytes, int *blocks);
 
 // p[0..n) = v (a kernel, so that it can be captured in a graph)
 hipError_t launch_fill(int *p, int n, int v, hipStream_t s);

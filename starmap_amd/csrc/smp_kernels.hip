@@ -124,10 +124,11 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     constexpr int EPC = 64 / DPE;           // elements per 64-lane DMA chunk
 
     auto hdr_nl = [&](int buf) { return reinterpret_cast<const int *>(ring + buf * BLOB_SZ)[0]; };
-    auto hdr_nt = [&](int buf) { return reinterpret_cast<const int *>(ring + buf * BLOB_SZ)[1]; };
+#if !SMP_SCALAR_RECORDS
     auto recs = [&](int buf) {
         return reinterpret_cast<const TgtRec *>(ring + buf * BLOB_SZ + BLOB_REC_OFF);
     };
+#endif
     auto ldrec = [&](int buf) {
         return reinterpret_cast<const DevLoad *>(ring + buf * BLOB_SZ + BLOB_LD_OFF);
     };
@@ -384,7 +385,7 @@ __global__ __launch_bounds__(CW * 64 + 128) void substep_kernel(const SubstepArg
 // problems.  Work units are (entry, tile) pairs, entries ordered by cost per
 // tile, heaviest first; every workgroup takes the next unit from a counter
 // until none is left, so uneven problems balance over the CUs.
-template <typename T, int NT, bool GEN, int CW>
+template <typename T, int NT, int NAPP, bool GEN, int CW>
 __global__ __launch_bounds__(CW * 64 + 128) void batch_kernel(const BatchLaunch<T> *bl) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int s_unit;
@@ -419,12 +420,10 @@ __global__ __launch_bounds__(CW * 64 + 128) void batch_kernel(const BatchLaunch<
 #pragma unroll
         for (int k = 0; k < (int)(sizeof(SubstepArgs<T>) / 4); ++k)
             dst[k] = __builtin_amdgcn_readfirstlane(src[k]);
-        // one or two half-step applications, each compiled with its count
-        // fixed (a runtime count costs the update its instruction schedule)
-        if (a.napp == 2)
-            substep_tile<T, NT, 2, GEN, CW>(a, u - ent[lo].unit0, smem_raw);
-        else
-            substep_tile<T, NT, 1, GEN, CW>(a, u - ent[lo].unit0, smem_raw);
+        // the launch's application count is compiled in (a runtime count
+        // costs the update its instruction schedule, and both tile paths in
+        // one kernel outgrow the instruction cache)
+        substep_tile<T, NT, NAPP, GEN, CW>(a, u - ent[lo].unit0, smem_raw);
     }
 }
 
@@ -472,32 +471,35 @@ hipError_t launch_substep(const SubstepArgs<T> &a, int nt, bool gen, int cw, siz
 
 // (the dynamic LDS limit of batch_kernel is raised by batch_occupancy, which
 // runs before any launch and outside graph capture)
-template <typename T, int CW>
+template <typename T, int NAPP, int CW>
 static const void *batch_fn_cw(int nt, bool gen) {
     if (nt <= 8)
-        return gen ? (const void *)batch_kernel<T, 8, true, CW> : (const void *)batch_kernel<T, 8, false, CW>;
-    return gen ? (const void *)batch_kernel<T, MAXT, true, CW>
-               : (const void *)batch_kernel<T, MAXT, false, CW>;
+        return gen ? (const void *)batch_kernel<T, 8, NAPP, true, CW>
+                   : (const void *)batch_kernel<T, 8, NAPP, false, CW>;
+    return gen ? (const void *)batch_kernel<T, MAXT, NAPP, true, CW>
+               : (const void *)batch_kernel<T, MAXT, NAPP, false, CW>;
 }
 
 template <typename T>
-static const void *batch_fn(int nt, bool gen, int cw) {
-    return cw == 8 ? batch_fn_cw<T, 8>(nt, gen) : batch_fn_cw<T, 4>(nt, gen);
+static const void *batch_fn(int nt, bool gen, int cw, int napp) {
+    if (napp == 2) return cw == 8 ? batch_fn_cw<T, 2, 8>(nt, gen) : batch_fn_cw<T, 2, 4>(nt, gen);
+    return cw == 8 ? batch_fn_cw<T, 1, 8>(nt, gen) : batch_fn_cw<T, 1, 4>(nt, gen);
 }
 
 template <typename T>
-hipError_t launch_batch(const BatchLaunch<T> *bl, int nt, bool gen, int cw, int grid,
+hipError_t launch_batch(const BatchLaunch<T> *bl, int nt, bool gen, int cw, int napp, int grid,
                         size_t lds_bytes, hipStream_t s) {
-    if (grid < 1 || lds_bytes > 160 * 1024 - 64 || (cw != 4 && cw != 8)) return hipErrorInvalidValue;
-    const void *f = batch_fn<T>(nt, gen, cw);
+    if (grid < 1 || lds_bytes > 160 * 1024 - 64 || (cw != 4 && cw != 8) || (napp != 1 && napp != 2))
+        return hipErrorInvalidValue;
+    const void *f = batch_fn<T>(nt, gen, cw, napp);
     void *args[] = {(void *)&bl};
     return hipLaunchKernel(f, dim3(grid), dim3(kthreads(cw)), args, lds_bytes, s);
 }
 
 template <typename T>
-hipError_t batch_occupancy(int nt, bool gen, int cw, size_t lds_bytes, int *blocks) {
-    if (cw != 4 && cw != 8) return hipErrorInvalidValue;
-    const void *f = batch_fn<T>(nt, gen, cw);
+hipError_t batch_occupancy(int nt, bool gen, int cw, int napp, size_t lds_bytes, int *blocks) {
+    if ((cw != 4 && cw != 8) || (napp != 1 && napp != 2)) return hipErrorInvalidValue;
+    const void *f = batch_fn<T>(nt, gen, cw, napp);
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64);
     if (e != hipSuccess) return e;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, f, kthreads(cw), lds_bytes);
@@ -663,9 +665,9 @@ hipError_t launch_reduce_final(const double *part, int blocks, double *out3, hip
                                          hipStream_t);                                         \
     template hipError_t launch_reduce<T>(const T *, const ReduceGeom &, double *, int,         \
                                          hipStream_t);                                         \
-    template hipError_t launch_batch<T>(const BatchLaunch<T> *, int, bool, int, int, size_t,   \
+    template hipError_t launch_batch<T>(const BatchLaunch<T> *, int, bool, int, int, int, size_t, \
                                         hipStream_t);                                          \
-    template hipError_t batch_occupancy<T>(int, bool, int, size_t, int *);
+    template hipError_t batch_occupancy<T>(int, bool, int, int, size_t, int *);
 
 SMP_INSTANTIATE(double)
 SMP_INSTANTIATE(float)

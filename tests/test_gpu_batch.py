@@ -67,7 +67,10 @@ def test_batch_bitwise_equal_single_handles(precision, cw, use_graph):
                 for j, (x, y) in enumerate(zip(states(s), ref[i])):
                     assert np.array_equal(x, y), (rep, i, j)
         tot, ker, launches, by, mcu, grid = b.timing()
-        assert launches == 2 * int(max(ns)) + 1 and grid >= 256 and tot > 0
+        # one launch per substep position, two where some problems take
+        # their last (one-application) odd substep while others go on
+        mixed = sum(1 for k in set(int(v) for v in ns) if k < int(max(ns)))
+        assert launches == 2 * int(max(ns)) + 1 + mixed and grid >= 256 and tot > 0
         assert mcu == sum(s.n_sys * int(np.prod(s.n)) * int(n) for s, n in zip(b.solvers, ns))
     finally:
         b.close()
