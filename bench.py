@@ -197,7 +197,8 @@ def batched_leg(fp64_plain_value, n_instances=64, check=True):
             "achieved_GBs": b_alg_bytes / (tot_ms / 1e3) / 1e9,
             "frac": b_alg_bytes / (tot_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
             "kernel_bytes_GBs": by / (tot_ms / 1e3) / 1e9,
-            "schedule": {"tile": list(tile), "blocks_per_cu": bpc, "cell_waves": cw,
+            "schedule": {"tile": tile if isinstance(tile, str) else list(tile),
+                         "blocks_per_cu": bpc, "cell_waves": cw,
                          "candidates_ms": {"%s/%d/%d" % c: v for c, v in times.items()}},
         }
         if check:

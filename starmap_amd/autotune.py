@@ -20,6 +20,17 @@ import numpy as np
 TILES = {1: [(255, 1, 1)],
          2: [(63, 4, 1), (127, 2, 1), (31, 8, 1)],
          3: [(15, 4, 4), (31, 4, 2), (15, 8, 2)]}
+# "fit": each handle of a batch its own tile, the one with the lowest
+# estimated cost for its grid -- tiles x slot size x the cost of a staged
+# element for the slot's row length (measured on the 2D P39 1000^2 lattice,
+# FP32, two workgroups per CU: rows of 16 elements copy slower than long
+# ones) -- so that small grids do not carry a nearly empty row or column of
+# tiles
+FIT = "fit"
+FIT_TILES = {1: [(255, 1, 1)],
+             2: [(63, 4, 1), (127, 2, 1), (31, 8, 1), (15, 16, 1)],
+             3: [(15, 4, 4), (31, 4, 2), (15, 8, 2)]}
+ROW_COST = {16: 1.33, 32: 1.15, 64: 1.0, 128: 0.92, 256: 0.9}
 BLOCKS = (1, 2)
 CELL_WAVES = (4, 8)
 _SLOT = 4095            # checkpoint slot used while tuning
@@ -64,14 +75,50 @@ def _store_cache(key, entry):
         pass
 
 
+def _dim(s):
+    return max(int(np.sum(np.asarray(s.n) > 1)), 1)
+
+
 def candidates(target):
-    dim = max(int(np.sum(np.asarray(s.n) > 1)) for s in _solvers(target))
-    return [(t, b, c) for t in TILES[max(dim, 1)] for b in BLOCKS for c in CELL_WAVES]
+    dim = max(_dim(s) for s in _solvers(target))
+    tiles = list(TILES[dim])
+    if hasattr(target, "solvers"):
+        tiles.append(FIT)
+    return [(t, b, c) for t in tiles for b in BLOCKS for c in CELL_WAVES]
+
+
+def staged_elements(s, tile):
+    """Elements one substep stages for handle s with this tile: tiles times
+    the slot size (tile + one-cell halo per axis)."""
+    ext = [int(v) for v in np.asarray(s.n) if v > 1] or [1]
+    tiles = 1
+    slot = 1
+    for e, n in enumerate(ext):
+        tiles *= -(-n // tile[e])
+        slot *= tile[e] + 1
+    return tiles * slot
+
+
+def fit_tiles(s):
+    """FIT_TILES of the handle's dimension, lowest estimated cost first."""
+    return sorted(FIT_TILES[_dim(s)], key=lambda t: staged_elements(s, t) * ROW_COST[t[0] + 1])
 
 
 def apply(target, tile, blocks_per_cu, cell_waves):
     for s in _solvers(target):
-        s.set_schedule(tile, blocks_per_cu, cell_waves)
+        if tile != FIT:
+            s.set_schedule(tile, blocks_per_cu, cell_waves)
+            continue
+        err = None
+        for t in fit_tiles(s):
+            try:
+                s.set_schedule(t, blocks_per_cu, cell_waves)
+                err = None
+                break
+            except Exception as e:       # a tile the coupling does not fit
+                err = e
+        if err is not None:
+            raise err
 
 
 def _time(target, steps):
@@ -90,7 +137,8 @@ def tune(target, steps=2, use_cache=True, verbose=False):
     if use_cache:
         hit = _load_cache().get(key)
         if hit:
-            best = (tuple(hit["tile"]), hit["blocks_per_cu"], hit["cell_waves"])
+            tile = hit["tile"] if hit["tile"] == FIT else tuple(hit["tile"])
+            best = (tile, hit["blocks_per_cu"], hit["cell_waves"])
             apply(target, *best)
             return best, {}
     sol = _solvers(target)
@@ -122,6 +170,7 @@ def tune(target, steps=2, use_cache=True, verbose=False):
         raise RuntimeError("autotune: no candidate schedule fits")
     best, _ = min(times.items(), key=lambda kv: kv[1])
     apply(target, *best)
-    _store_cache(key, {"tile": list(best[0]), "blocks_per_cu": best[1], "cell_waves": best[2],
+    _store_cache(key, {"tile": best[0] if best[0] == FIT else list(best[0]),
+                       "blocks_per_cu": best[1], "cell_waves": best[2],
                        "ms": {"%s/%d/%d" % c: v for c, v in times.items()}})
     return best, times

@@ -154,3 +154,26 @@ def test_batch_autotune():
             assert all(np.array_equal(x, y) for x, y in zip(states(s), ref[i]))
     finally:
         b.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_batch_mixed_tiles_bitwise(precision):
+    """Each handle on its own tile shape (the autotuner's "fit" schedule):
+    one batched launch then mixes tile shapes and slot sizes, and every
+    problem still equals its single-handle run bit for bit."""
+    pars = small_sweep(n=6, side_div=2, tfinal=0.05)
+    ref = single_runs(pars, precision)
+    b = BatchSolver(pars, precision=precision)
+    try:
+        b.prepare()
+        autotune.apply(b, autotune.FIT, 2, 4)
+        tiles = {tuple(s.get_schedule()[0]) for s in b.solvers}
+        print("batch fit", precision, [(p["n"][1], p["n_mom"]) for p in pars], sorted(tiles), flush=True)
+        assert len(tiles) > 1
+        dts, ns = b.prepare()
+        b.step(ns, use_graph=True)
+        for i, s in enumerate(b.solvers):
+            for j, (x, y) in enumerate(zip(states(s), ref[i])):
+                assert np.array_equal(x, y), (i, j)
+    finally:
+        b.close()
