@@ -319,16 +319,6 @@ __global__ void dedsig_kernel(const T *sig, T *dE, int64_t count, double dt) {
     }
 }
 
-// dst = src on the slabs of the components in list (elements [0, slab) of
-// each): grid.y over the list
-template <typename T>
-__global__ void copy_list_kernel(T *dst, const T *src, const int32_t *list, int64_t slab) {
-    const int64_t base = (int64_t)glb(list)[blockIdx.y] * slab;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < slab;
-         i += (int64_t)gridDim.x * blockDim.x)
-        glb(dst)[base + i] = glb(src)[base + i];
-}
-
 template <typename T>
 __global__ void copy_kernel(T *dst, const T *src, int64_t count) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
@@ -388,16 +378,6 @@ hipError_t launch_dedsig(const T *sig, T *dE, int64_t count, double dt, hipStrea
 }
 
 template <typename T>
-hipError_t launch_copy_list(T *dst, const T *src, const int32_t *list, int n_list, int64_t slab,
-                            hipStream_t s) {
-    if (n_list <= 0 || slab <= 0) return hipSuccess;
-    const int64_t blocks = std::min<int64_t>((slab + 255) / 256, 1024);
-    hipLaunchKernelGGL(copy_list_kernel<T>, dim3((unsigned)blocks, (unsigned)n_list), dim3(256), 0, s,
-                       dst, src, list, slab);
-    return hipGetLastError();
-}
-
-template <typename T>
 hipError_t launch_copy(T *dst, const T *src, int64_t count, hipStream_t s) {
     if (count <= 0) return hipSuccess;
     const int64_t blocks = std::min<int64_t>((count + 255) / 256, 1 << 16);
@@ -409,8 +389,7 @@ hipError_t launch_copy(T *dst, const T *src, int64_t count, hipStream_t s) {
     template hipError_t launch_adj_update<T>(const AdjArgs<T> &, bool, hipStream_t); \
     template hipError_t launch_adj_transpose<T>(const AdjArgs<T> &, hipStream_t);  \
     template hipError_t launch_copy<T>(T *, const T *, int64_t, hipStream_t);    \
-    template hipError_t launch_dedsig<T>(const T *, T *, int64_t, double, hipStream_t); \
-    template hipError_t launch_copy_list<T>(T *, const T *, const int32_t *, int, int64_t, hipStream_t);
+    template hipError_t launch_dedsig<T>(const T *, T *, int64_t, double, hipStream_t);
 
 SMP_ADJ_INSTANTIATE(double)
 SMP_ADJ_INSTANTIATE(float)

@@ -108,7 +108,7 @@ struct smp_problem {
     // discrete adjoint: lambda, mu (state layout), dJ/dsigma per component
     // (FP64), tables of the adjoint kernels, segment checkpoints and the
     // states recorded while a segment is recomputed
-    void *lam = nullptr, *mu = nullptr, *G = nullptr, *work = nullptr, *work2 = nullptr;
+    void *lam = nullptr, *mu = nullptr, *G = nullptr;
     hipStream_t adj_stream = nullptr;   // the adjoint's recomputation (default priority)
     int adj_ngrp[2] = {0, 0};     // odd, even: grid types, list ranges, codes
     int32_t adj_grp0[2][9] = {};
@@ -736,6 +736,8 @@ smp::SubstepArgs<T> make_args(smp_problem *h, int kind, double dt, int napp, int
     smp::SubstepArgs<T> a;
     std::memset(&a, 0, sizeof(a));
     a.U = (T *)(U ? U : h->U);
+    a.out = a.U;
+    a.src = a.U;
     a.sink = (T *)h->sink;
     a.slab = h->slab;
     a.pitch0 = h->pitch0;
@@ -1035,7 +1037,7 @@ int adjoint_tables(smp_problem *h, double dt) {
 
 int adjoint_buffers(smp_problem *h) {
     const size_t st = (size_t)h->slab * h->elem * h->geo.n_sys;
-    for (void **p : {&h->lam, &h->mu, &h->work, &h->work2}) {
+    for (void **p : {&h->lam, &h->mu}) {
         if (!*p) {
             int rc = h->dalloc(p, st);
             if (rc) return rc;
@@ -1084,8 +1086,9 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
     }
     // records of two segments' recomputations (the next segment is
     // recomputed while the current one is stepped back): the targets of
-    // every launch before it (2*every+1 launches); the other parity's slabs
-    // of a record are never written and stay zero
+    // every launch (2*every+1 launches) after it, written there by the
+    // launch itself; the other parity's slabs of a record are never written
+    // and stay zero
     const int64_t rec_per_set = 2 * (int64_t)every + 1;
     while ((int64_t)h->adj_rec.size() < 2 * rec_per_set) {
         void *p = nullptr;
@@ -1147,14 +1150,14 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
     // reverse sweep, one segment at a time.  The segment's launches are the
     // forward's (odd first, then even and fused odd per step, odd last):
     // launch L updates the targets of one parity from the other parity with
-    // napp applications.  The recomputation records the targets of each
-    // launch before it; the sources of launch L are the targets of L+1
-    // before L+1 (the current state for the last launch).  The adjoint
-    // steps back launch by launch: the update of L's targets, then the
-    // transposed flux into L's sources.
-    // Two buffer sets (records and a running state): segment sg uses set
-    // sg % 2.  The recomputation runs on its own stream one segment ahead
-    // of the reverse sweep, which runs on the handle's stream.
+    // napp applications.  The recomputation writes the targets of launch L
+    // to record L, so L's targets before L are in record L-2 and its sources
+    // in record L-1 (the segment's checkpoint where the index is negative).
+    // The adjoint steps back launch by launch: the update of L's targets,
+    // then the transposed flux into L's sources.
+    // Two record sets: segment sg uses set sg % 2.  The recomputation runs
+    // on its own stream one segment ahead of the reverse sweep, which runs
+    // on the handle's stream.
     if (!h->adj_stream) HIP_TRY(hipStreamCreateWithFlags(&h->adj_stream, hipStreamNonBlocking));
     hipStream_t rs = h->adj_stream;
     hipEvent_t ev_fwd, ev_rc[2], ev_rev[2];
@@ -1165,33 +1168,34 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
     }
     HIP_TRY(hipEventRecord(ev_fwd, h->stream));
     HIP_TRY(hipStreamWaitEvent(rs, ev_fwd, 0));
-    void *const work_of[2] = {h->work, h->work2};
-    smp::SubstepArgs<T> f_odd_first[2], f_even[2], f_odd_fused[2], f_odd_last[2];
-    for (int q = 0; q < 2; ++q) {
-        f_odd_first[q] = make_args<T>(h, 0, dt, 1, 0, 0, work_of[q]);
-        f_even[q] = make_args<T>(h, 1, dt, 2, 0, 1, work_of[q]);
-        f_odd_fused[q] = make_args<T>(h, 0, dt, 2, 1, 0, work_of[q]);
-        f_odd_last[q] = make_args<T>(h, 0, dt, 1, 1, 1, work_of[q]);
-    }
+    const smp::SubstepArgs<T> f_odd_first = make_args<T>(h, 0, dt, 1, 0, 0);
+    const smp::SubstepArgs<T> f_even = make_args<T>(h, 1, dt, 2, 0, 1);
+    const smp::SubstepArgs<T> f_odd_fused = make_args<T>(h, 0, dt, 2, 1, 0);
+    const smp::SubstepArgs<T> f_odd_last = make_args<T>(h, 0, dt, 1, 1, 1);
     auto seg_launches = [&](int64_t sg) { return 2 * std::min<int64_t>(every, n_steps - sg * every) + 1; };
-    auto launch_of = [&](int q, int64_t L, int64_t nl) -> const smp::SubstepArgs<T> & {
-        if (L == 0) return f_odd_first[q];
-        if (L == nl - 1) return f_odd_last[q];
-        return L % 2 ? f_even[q] : f_odd_fused[q];
+    auto launch_of = [&](int64_t L, int64_t nl) -> const smp::SubstepArgs<T> & {
+        if (L == 0) return f_odd_first;
+        if (L == nl - 1) return f_odd_last;
+        return L % 2 ? f_even : f_odd_fused;
     };
     auto rec = [&](int q, int64_t L) { return h->adj_rec[q * rec_per_set + L]; };
-    // recompute segment sg on the recomputation stream
+    // the state of segment sg after its launch m (m < 0: the segment's
+    // checkpoint), for the parity launch m targets
+    auto after = [&](int64_t sg, int64_t m) { return m >= 0 ? rec((int)(sg % 2), m) : h->adj_seg[sg]; };
+    // recompute segment sg on the recomputation stream, out of place: launch
+    // L reads its targets after launch L-2 and its sources after launch L-1
+    // and writes its targets to record L
     auto recompute = [&](int64_t sg) -> int {
         const int q = (int)(sg % 2);
         const int64_t nl = seg_launches(sg);
-        HIP_TRY(hipMemcpyAsync(work_of[q], h->adj_seg[sg], st, hipMemcpyDeviceToDevice, rs));
         for (int64_t L = 0; L < nl; ++L) {
             const int kind = L % 2;   // 0: odd targets
-            HIP_TRY(smp::launch_copy_list<T>((T *)rec(q, L), (const T *)work_of[q],
-                                             kind ? list_even : list_odd, kind ? nl_even : nl_odd,
-                                             h->slab, rs));
-            HIP_TRY(smp::launch_substep<T>(launch_of(q, L, nl), h->plan[kind].max_terms, h->gen[kind],
-                                           h->cell_waves, h->lds_bytes[kind], rs));
+            smp::SubstepArgs<T> a = launch_of(L, nl);
+            a.U = (T *)after(sg, L - 2);
+            a.src = (const T *)after(sg, L - 1);
+            a.out = (T *)rec(q, L);
+            HIP_TRY(smp::launch_substep<T>(a, h->plan[kind].max_terms, h->gen[kind], h->cell_waves,
+                                           h->lds_bytes[kind], rs));
         }
         HIP_TRY(hipEventRecord(ev_rc[q], rs));
         return SMP_OK;
@@ -1213,9 +1217,9 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
         // ends with it as a launch of its own
         for (int64_t L = nl - 1; L >= 0; --L) {
             const int kind = L % 2;
-            const smp::SubstepArgs<T> &f = launch_of(q, L, nl);
-            const void *U0 = rec(q, L);
-            const void *U = L + 1 < nl ? rec(q, L + 1) : work_of[q];
+            const smp::SubstepArgs<T> &f = launch_of(L, nl);
+            const void *U0 = after(sg, L - 2);   // L's targets before L
+            const void *U = after(sg, L - 1);    // L's sources
             const int32_t *tl = kind ? list_even : list_odd, *sl = kind ? list_odd : list_even;
             const int ntl = kind ? nl_even : nl_odd, nsl = kind ? nl_odd : nl_even;
             HIP_TRY(smp::launch_adj_update<T>(adj_args(U, U0, tl, ntl, f.napp, f.fslot[0], f.fslot[1]),

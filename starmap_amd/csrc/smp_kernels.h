@@ -23,7 +23,9 @@ constexpr int kthreads(int cw) { return cw * 64 + 128; }
 
 template <typename T>
 struct SubstepArgs {
-    T *U;
+    T *U;                     // target values read
+    T *out;                   // target values written (U for an update in place)
+    const T *src;             // source components (U for an update in place)
     T *sink;                  // 256 elements written by lanes without a target cell
     int64_t slab;             // elements per component
     int32_t pitch0, ext1;     // storage index = i0 + pitch0*(i1 + ext1*i2)
@@ -225,11 +227,5 @@ hipError_t launch_adj_transpose(const AdjArgs<T> &a, hipStream_t s);
 // dst[i] = src[i] for a contiguous range of elements
 template <typename T>
 hipError_t launch_copy(T *dst, const T *src, int64_t count, hipStream_t s);
-
-// the slabs of the components in list (device array of n_list indices)
-// copied from src to dst
-template <typename T>
-hipError_t launch_copy_list(T *dst, const T *src, const int32_t *list, int n_list, int64_t slab,
-                            hipStream_t s);
 
 }  // namespace smp

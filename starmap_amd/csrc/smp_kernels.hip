@@ -202,7 +202,7 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
             const int comp = rec & 0xFFFFF;
             const int slot = (rec >> 20) & 0xFF;
             const int m = (rec >> 28) & 7;   // bit e: cell-centred on kernel axis e
-            SMP_GLOBAL const unsigned *cb = glb(reinterpret_cast<const unsigned *>(a.U + (int64_t)comp * a.slab));
+            SMP_GLOBAL const unsigned *cb = glb(reinterpret_cast<const unsigned *>(a.src + (int64_t)comp * a.slab));
             unsigned char *dst = reinterpret_cast<unsigned char *>(lds + slot * S);
             if (low_inner) {
                 // every shifted coordinate is x-1: one uniform base shift
@@ -238,7 +238,7 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     // wait for a prefetched target value with an exact count instead of
     // draining every load and store in flight.
     SMP_GLOBAL const char *ug_ld = glb(reinterpret_cast<const char *>(a.U)) + (in_store ? gidx : 0) * sizeof(T);
-    SMP_GLOBAL char *ug_st = glb(reinterpret_cast<char *>(a.U)) + gidx * sizeof(T);
+    SMP_GLOBAL char *ug_st = glb(reinterpret_cast<char *>(a.out)) + gidx * sizeof(T);
     SMP_GLOBAL T *sink = glb(a.sink);
     // The cell waves' records of group gq: with SMP_SCALAR_RECORDS, read
     // with scalar loads from the plan in global memory (every workgroup

@@ -148,7 +148,14 @@ def test_batch_autotune():
     try:
         dts, ns = b.prepare()
         best, times = autotune.tune(b, steps=1, use_cache=False)
-        assert times and all(x.get_schedule() == best for x in b.solvers)
+        assert times
+        for x in b.solvers:
+            tile, bpc, cw = x.get_schedule()
+            assert (bpc, cw) == best[1:]
+            if best[0] == autotune.FIT:      # each problem on its own tile
+                assert tile in [tuple(t) for t in autotune.fit_tiles(x)]
+            else:
+                assert tile == best[0]
         b.step(ns)
         for i, s in enumerate(b.solvers):
             assert all(np.array_equal(x, y) for x, y in zip(states(s), ref[i]))
