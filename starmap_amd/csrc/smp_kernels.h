@@ -18,8 +18,16 @@ constexpr int BLOB_SZ = 3072;
 constexpr int BLOB_REC_OFF = 16;
 constexpr int BLOB_LD_OFF = BLOB_REC_OFF + GMAX * (int)sizeof(TgtRec);
 constexpr int CPS_MAX = 16;   // DMA chunks per LDS slot
-// threads of a substep workgroup: cw cell waves (4 or 8) + 2 copy waves
-constexpr int kthreads(int cw) { return cw * 64 + 128; }
+// Waves copying sources into LDS beside cw cell waves (4 or 8).  The copies
+// are issued 256 bytes per instruction, so more copy waves keep more bytes
+// in flight: 8 beside 8 cell waves (one workgroup of 16 waves per CU), 4
+// beside 4 (two workgroups of 8 waves per CU).  The batched kernel keeps 4
+// in both cases (with 8 cell waves its FP64 variants need more than the 128
+// VGPRs a 16-wave workgroup leaves).
+constexpr int copy_waves(int cw) { return cw == 8 ? 8 : 4; }
+constexpr int batch_copy_waves(int) { return 4; }
+constexpr int kthreads(int cw) { return (cw + copy_waves(cw)) * 64; }
+constexpr int batch_threads(int cw) { return (cw + batch_copy_waves(cw)) * 64; }
 
 template <typename T>
 struct SubstepArgs {

@@ -63,11 +63,12 @@ __device__ __forceinline__ SMP_GLOBAL P *glb(P *p) {
 }
 
 // One workgroup = one spatial tile of up to 256 cells.  The first CW waves
-// (cell waves) own one cell per thread and update the targets; the last two
+// (cell waves) own one cell per thread and update the targets; the last NCW
 // waves copy: source tiles and the group records go from HBM straight into
-// LDS (global_load_lds, no registers).  While the cell waves update the targets of group g from LDS,
-// the copy wave brings in the source tiles of group g+1 and the records of
-// group g+2, and the cell waves load the target values of group g+1; one
+// LDS (global_load_lds, no registers).  While the cell waves update the
+// targets of group g from LDS, the copy waves bring in the source tiles of
+// group g+1 and the records of group g+2, and the cell waves load the
+// target values of group g+1; one
 // barrier per group.  Only the copy waves have copies in flight, so the cell
 // waves never wait on them before reading LDS.  The schedule never lets group
 // g+1 copy into a slot that group g reads.
@@ -78,7 +79,7 @@ __device__ __forceinline__ SMP_GLOBAL P *glb(P *p) {
 // space (an entry of the batched kernel, read with scalar loads).
 // CW cell waves (4 or 8): with 8, two waves per SIMD share each cell, each
 // updating half of a group's targets.
-template <typename T, int NT, int NAPP, bool GEN, int CW, typename AT>
+template <typename T, int NT, int NAPP, bool GEN, int CW, int NCW, typename AT>
 __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsigned char *smem_raw) {
     constexpr int NCT = CW * 64;              // cell threads
     constexpr int TPT = GMAX * 4 / CW;        // targets of a group per cell thread
@@ -190,14 +191,14 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
         // LDS read behind an outstanding copy would have to wait for it):
         // comp | slot << 20 | (origin-shift bits) << 28
         const int nl = __builtin_amdgcn_readfirstlane(hdr_nl(buf));
-        const int cw = (tid >> 6) - CW;  // copy wave 0 or 1: loads of its parity
+        const int cw = (tid >> 6) - CW;  // copy wave cw: every NCW-th load
         int myrec = 0;
         if (lane < nl) {
             const DevLoad &d = ldrec(buf)[lane];
             myrec = d.comp | (d.slot << 20) |
                     (((d.org[0] & 1) | ((d.org[1] & 1) << 1) | ((d.org[2] & 1) << 2)) << 28);
         }
-        for (int li = cw; li < nl; li += 2) {
+        for (int li = cw; li < nl; li += NCW) {
             const int rec = __builtin_amdgcn_readlane(myrec, li);
             const int comp = rec & 0xFFFFF;
             const int slot = (rec >> 20) & 0xFF;
@@ -374,11 +375,11 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
 }
 
 template <typename T, int NT, int NAPP, bool GEN, int CW>
-__global__ __launch_bounds__(CW * 64 + 128) void substep_kernel(const SubstepArgs<T> a) {
+__global__ __launch_bounds__(kthreads(CW)) void substep_kernel(const SubstepArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= a.n_tiles) return;
-    substep_tile<T, NT, NAPP, GEN, CW>(a, tile, smem_raw);
+    substep_tile<T, NT, NAPP, GEN, CW, copy_waves(CW)>(a, tile, smem_raw);
 }
 
 // Batched persistent kernel: one launch runs a substep of many independent
@@ -386,7 +387,7 @@ __global__ __launch_bounds__(CW * 64 + 128) void substep_kernel(const SubstepArg
 // tile, heaviest first; every workgroup takes the next unit from a counter
 // until none is left, so uneven problems balance over the CUs.
 template <typename T, int NT, int NAPP, bool GEN, int CW>
-__global__ __launch_bounds__(CW * 64 + 128) void batch_kernel(const BatchLaunch<T> *bl) {
+__global__ __launch_bounds__(batch_threads(CW)) void batch_kernel(const BatchLaunch<T> *bl) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int s_unit;
     const int n_units = bl->n_units;
@@ -423,7 +424,7 @@ __global__ __launch_bounds__(CW * 64 + 128) void batch_kernel(const BatchLaunch<
         // the launch's application count is compiled in (a runtime count
         // costs the update its instruction schedule, and both tile paths in
         // one kernel outgrow the instruction cache)
-        substep_tile<T, NT, NAPP, GEN, CW>(a, u - ent[lo].unit0, smem_raw);
+        substep_tile<T, NT, NAPP, GEN, CW, batch_copy_waves(CW)>(a, u - ent[lo].unit0, smem_raw);
     }
 }
 
@@ -493,7 +494,7 @@ hipError_t launch_batch(const BatchLaunch<T> *bl, int nt, bool gen, int cw, int 
         return hipErrorInvalidValue;
     const void *f = batch_fn<T>(nt, gen, cw, napp);
     void *args[] = {(void *)&bl};
-    return hipLaunchKernel(f, dim3(grid), dim3(kthreads(cw)), args, lds_bytes, s);
+    return hipLaunchKernel(f, dim3(grid), dim3(batch_threads(cw)), args, lds_bytes, s);
 }
 
 template <typename T>
@@ -502,7 +503,7 @@ hipError_t batch_occupancy(int nt, bool gen, int cw, int napp, size_t lds_bytes,
     const void *f = batch_fn<T>(nt, gen, cw, napp);
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64);
     if (e != hipSuccess) return e;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, f, kthreads(cw), lds_bytes);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, f, batch_threads(cw), lds_bytes);
 }
 
 hipError_t launch_fill(int *p, int n, int v, hipStream_t s) {
