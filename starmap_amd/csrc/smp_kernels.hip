@@ -79,7 +79,7 @@ __device__ __forceinline__ SMP_GLOBAL P *glb(P *p) {
 // space (an entry of the batched kernel, read with scalar loads).
 // CW cell waves (4 or 8): with 8, two waves per SIMD share each cell, each
 // updating half of a group's targets.
-template <typename T, int NT, int NAPP, bool GEN, int CW, int NCW, typename AT>
+template <typename T, int NT, int NAPP, bool GEN, int CW, int NCW, bool WARM, typename AT>
 __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsigned char *smem_raw) {
     constexpr int NCT = CW * 64;              // cell threads
     constexpr int TPT = GMAX * 4 / CW;        // targets of a group per cell thread
@@ -321,12 +321,32 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
         }
     };
 
+    // The cell waves read their target records with scalar loads, and a
+    // wait for a scalar load also waits for their LDS reads in flight; a
+    // tile's records are far larger than the scalar cache.  With WARM (the
+    // batched kernel, whose small tiles change problem and plan often), the
+    // last copy wave reads one dword of every 64-byte line of a group's
+    // records two phases before the cell waves use them, so that their loads
+    // hit.  The plain kernel's tuned schedules hide the misses, and the
+    // extra loads cost them 1-2%.
+    const bool warmer = WARM && SMP_SCALAR_RECORDS &&
+                        __builtin_amdgcn_readfirstlane(tid >> 6) == CW + NCW - 1;
+    auto warm_records = [&](int gq) {
+        const SMP_CONST int *p = (const SMP_CONST int *)(a.blobs + (int64_t)gq * BLOB_SZ);
+        int acc = 0;
+#pragma unroll
+        for (int k = 0; k < (BLOB_REC_OFF + GMAX * (int)sizeof(TgtRec) + 63) / 64; ++k) acc += p[k * 16];
+        asm volatile("" : : "s"(acc));
+    };
+
     // ---- copy waves ----------------------------------------------------------
     // prologue (two barriers), then one barrier per group: the sources of
     // group g+1 and the records of group g+2 while the cells update group g
     if (copier) {
         if (ng > 0) blob_dma(0, 0);
         if (ng > 1) blob_dma(1, 1);
+        if (warmer && ng > 0) warm_records(0);
+        if (warmer && ng > 1) warm_records(1);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         if (ng > 0) src_dma(0);
@@ -335,6 +355,7 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
         for (int gq = 0; gq < ng; ++gq) {
             if (gq + 1 < ng) src_dma((gq + 1) % 3);
             if (gq + 2 < ng) blob_dma(gq + 2, (gq + 2) % 3);
+            if (warmer && gq + 2 < ng) warm_records(gq + 2);
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
         }
@@ -379,7 +400,7 @@ __global__ __launch_bounds__(kthreads(CW)) void substep_kernel(const SubstepArgs
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= a.n_tiles) return;
-    substep_tile<T, NT, NAPP, GEN, CW, copy_waves(CW)>(a, tile, smem_raw);
+    substep_tile<T, NT, NAPP, GEN, CW, copy_waves(CW), false>(a, tile, smem_raw);
 }
 
 // Batched persistent kernel: one launch runs a substep of many independent
@@ -424,7 +445,7 @@ __global__ __launch_bounds__(batch_threads(CW)) void batch_kernel(const BatchLau
         // the launch's application count is compiled in (a runtime count
         // costs the update its instruction schedule, and both tile paths in
         // one kernel outgrow the instruction cache)
-        substep_tile<T, NT, NAPP, GEN, CW, batch_copy_waves(CW)>(a, u - ent[lo].unit0, smem_raw);
+        substep_tile<T, NT, NAPP, GEN, CW, batch_copy_waves(CW), true>(a, u - ent[lo].unit0, smem_raw);
     }
 }
 
