@@ -68,13 +68,13 @@ __device__ __forceinline__ SMP_GLOBAL P *glb(P *p) {
 // LDS (global_load_lds, no registers).  While the cell waves update the
 // targets of group g from LDS, the copy waves bring in the source tiles of
 // group g+1 and the records of group g+2, and the cell waves load the
-// target values of group g+1; one
-// barrier per group.  Only the copy waves have copies in flight, so the cell
-// waves never wait on them before reading LDS.  The schedule never lets group
-// g+1 copy into a slot that group g reads.
+// target values of group g+1; one barrier per group.  Only the copy waves
+// have copies in flight, so the cell waves never wait on them before reading
+// LDS.  The schedule never lets group g+1 copy into a slot that group g
+// reads.
 //
-// NAPP = 0 takes the number of half-step applications from a.napp (the
-// batched kernel, whose work units mix substeps of different kinds).
+// NAPP: the number of half-step applications, compiled in (the batched
+// kernel has one launch per application count).
 // AT: SubstepArgs<T> (kernel argument) or the same in the constant address
 // space (an entry of the batched kernel, read with scalar loads).
 // CW cell waves (4 or 8): with 8, two waves per SIMD share each cell, each
