@@ -110,14 +110,31 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     const int t1 = b % a.ntile[1];
     const int t2 = b / a.ntile[1];
     const int o0 = t0 * a.tile[0], o1 = t1 * a.tile[1], o2 = t2 * a.tile[2];
-    const int ncell = a.tile[0] * a.tile[1] * a.tile[2];
-    const bool has_cell = tid < NCT && ctid < ncell;
-    const int c0 = ctid % a.tile[0];
-    const int c1 = (ctid / a.tile[0]) % a.tile[1];
-    const int c2 = ctid / (a.tile[0] * a.tile[1]);
+    // The cell of a cell thread.  Where lds[0]*tile[1]*tile[2] <= 256 (every
+    // tile shape the library picks: lds[0] = tile[0] + 1), thread ctid takes
+    // the slot column ctid mod lds[0], so a wave's LDS reads are consecutive
+    // elements; numbered along the tile instead, the threads would skip the
+    // slot's halo column once per tile row, and two threads of one read pass
+    // would meet in the same bank.  A thread on a column or row without a
+    // cell reads the address of the nearest cell (a broadcast, in the slot).
+    const bool by_slot = a.lds[0] * a.tile[1] * a.tile[2] <= 256;
+    int c0, c1, c2;
+    bool has_cell;
+    if (by_slot) {
+        const int r = ctid >> a.lds0_shift;
+        c0 = ctid & (a.lds[0] - 1);
+        c1 = r % a.tile[1];
+        c2 = r / a.tile[1];
+        has_cell = tid < NCT && c0 < a.tile[0] && c2 < a.tile[2];
+    } else {
+        c0 = ctid % a.tile[0];
+        c1 = (ctid / a.tile[0]) % a.tile[1];
+        c2 = ctid / (a.tile[0] * a.tile[1]);
+        has_cell = tid < NCT && ctid < a.tile[0] * a.tile[1] * a.tile[2];
+    }
     const int g0 = o0 + c0, g1 = o1 + c1, g2 = o2 + c2;
     const bool in_store = has_cell && g0 < a.E[0] && g1 < a.E[1] && g2 < a.E[2];
-    const int lbase = c0 + a.lds[0] * (c1 + a.lds[1] * c2);
+    const int lbase = min(c0, a.tile[0] - 1) + a.lds[0] * (min(c1, a.tile[1] - 1) + a.lds[1] * min(c2, a.tile[2] - 1));
     const int64_t gidx = g0 + (int64_t)a.pitch0 * (g1 + (int64_t)a.ext1 * g2);
     const int S = a.slot_elems;
     const int ng = a.ngroups;
