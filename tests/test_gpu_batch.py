@@ -99,7 +99,10 @@ def test_schedule_change_keeps_every_bit():
     prob = P.lattice_2d(n=60, n_mom=9, tfinal=0.3)
     par = I.starmap_init(prob)
     res = []
-    for tile, bpc, cw in [(None, 0, 0), ((127, 2), 1, 8), ((31, 8), 2, 4), ((63, 4), 1, 8)]:
+    # 31x7: threads by slot column, a row of threads without a cell; 15x17:
+    # 16x17 slot elements above 256, threads numbered along the tile
+    for tile, bpc, cw in [(None, 0, 0), ((127, 2), 1, 8), ((31, 8), 2, 4), ((63, 4), 1, 8),
+                          ((31, 7), 1, 8), ((15, 17), 2, 4)]:
         s = HipSolver(par, precision="fp64")
         try:
             dt = s.prepare()
