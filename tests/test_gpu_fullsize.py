@@ -1,0 +1,89 @@
+"""Full-size checks of the HIP path where the oracle is too slow: exact
+linearity.
+
+For fixed materials and filters the scheme is linear in the state and the
+source, and scaling by 2 commutes with rounding (the values here are far from
+overflow and from the subnormal range).  So a run from 2*U0 with source 2*Q
+equals twice the run from U0 with Q, bit for bit, after any number of steps,
+at every size.  Both runs start from the same seeded random state in every
+component, so every component carries values.
+
+This catches reads of anything that is not state (coefficient fields, stale
+or uninitialised memory, the sink), races between groups, tiles or streams,
+and non-linear mistakes, at the full sizes: the 2D P39 lattice on 1000x1000
+(820 components, 4016 tiles per launch) and the 3D P7 Gaussian on 200^3.  It
+does not catch a wrong linear stencil, which the oracle parity cases check at
+smaller sizes (P39 among them).
+
+Checked through the device moment sums of every component (sums of squares
+and of squared neighbour differences exactly 4x, in FP64) and a sample of
+whole fields (exactly 2x).
+"""
+import inspect
+
+import numpy as np
+import pytest
+
+from starmap_amd import init as I
+from starmap_amd import problems as P
+from starmap_amd.observables import moment_terms
+from starmap_amd.solver import HipSolver
+
+pytestmark = pytest.mark.gpu
+
+
+def twice(f):
+    """2*f with f's number of arguments (which tells the set-up whether the
+    source depends on time)."""
+    n = len(inspect.signature(f).parameters)
+    g = lambda *x: 2.0 * np.asarray(f(*x), dtype=np.float64)
+    return {2: lambda a, b: g(a, b), 3: lambda a, b, c: g(a, b, c),
+            4: lambda a, b, c, d: g(a, b, c, d)}[n]
+
+
+def doubled(prob):
+    q = dict(prob)
+    if callable(prob.get("source")):
+        q["source"] = twice(prob["source"])
+    return q
+
+
+def check_linear(prob, precision, steps, seed=11):
+    a = HipSolver(I.starmap_init(prob), precision=precision)
+    b = HipSolver(I.starmap_init(doubled(prob)), precision=precision)
+    try:
+        dt = a.prepare()
+        assert b.prepare() == dt
+        rng = np.random.default_rng(seed)
+        for j in range(a.n_sys):
+            u = rng.random(a.shape(j)) - 0.5
+            if precision == "fp32":
+                u = u.astype(np.float32).astype(np.float64)
+            a.set_state(j, u)
+            b.set_state(j, 2.0 * u)
+        a.step(dt, steps)
+        b.step(dt, steps)
+        sa, da = moment_terms(a)
+        sb, db = moment_terms(b)
+        print("linearity", precision, a.n_sys, "components, sum of squares of component 0:", sa[0],
+              flush=True)
+        assert np.all(sa > 0)
+        assert np.array_equal(sb, 4.0 * sa)
+        assert np.array_equal(db, 4.0 * da)
+        for j in sorted({0, 1, 2, a.n_sys // 2, a.n_sys - 2, a.n_sys - 1}):
+            assert np.array_equal(b.get_state(j), 2.0 * a.get_state(j)), j
+    finally:
+        a.close()
+        b.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_lattice_p39_1000_linear(precision):
+    """2D P39 lattice, 1000x1000, extrapolation boundaries, substep filter,
+    source in the centre block: three steps (seven launches)."""
+    check_linear(P.lattice_2d(n=1000, n_mom=39, tfinal=3.2), precision, 3)
+
+
+def test_gauss3d_p7_200_linear():
+    """3D P7, 200^3 periodic, anisotropic scattering: two steps in FP64."""
+    check_linear(P.gauss_periodic_3d(n=200, n_mom=7, tfinal=0.75), "fp64", 2)
