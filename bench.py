@@ -336,7 +336,8 @@ def main():
     ap.add_argument("--n", type=int, default=1000, help="grid side")
     ap.add_argument("--n-mom", type=int, default=39)
     ap.add_argument("--precision", choices=["fp64", "fp32", "both"], default="both")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    # about 12 s of the oracle's stepping on 16 host threads
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--c3-steps", type=int, default=50,
                     help="steps of the 3D P7 periodic leg (0 skips it)")
