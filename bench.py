@@ -232,16 +232,22 @@ def adjoint_leg(n=512, n_mom=15, steps=2000, every=50, fd_cells=8, seed=2027):
     every 50 steps, FP64 gradient accumulation) of J = h^2 * sum of U1(T)
     over x > 0.8; per-cell dJ/dsigma_a, dJ/dsigma_s0 against FP64 central
     differences (relative step 1e-6) at seeded cells."""
-    from starmap_amd import init
+    from starmap_amd import autotune, init
     from starmap_amd import problems as P
     from starmap_amd.adjoint import Adjoint, objective_value, strip_objective
     prob = P.adjoint_grf(n=n, n_mom=n_mom, steps=steps)
     par = init.starmap_init(prob)
     log("adjoint: %dx%d P%d, %d steps" % (n, n, n_mom, steps))
     s = Adjoint(par, precision="fp32")
+    schedule = None
     try:
         mask = strip_objective(s, xmin=0.8)
         dt = s.prepare()
+        if TUNE:
+            # the forward on its fastest schedule (the recomputation uses it too)
+            (tile, bpc, cw), _ = autotune.tune(s, steps=2, use_cache=False)
+            schedule = {"tile": list(tile), "blocks_per_cu": bpc, "cell_waves": cw}
+            log("  autotuner: tile %s, %d workgroups/CU budget, %d cell waves" % (tile, bpc, cw))
         s.step(dt, steps)
         fwd_ms = s.timing()[0]
         J32 = objective_value(s, mask)
@@ -274,6 +280,7 @@ def adjoint_leg(n=512, n_mom=15, steps=2000, every=50, fd_cells=8, seed=2027):
         "J": J32,
         "update_alg_bytes_per_launch": upd_bytes,
         "launches_per_step": {"forward": 2, "recompute": 2, "adjoint_update": 2},
+        "schedule": schedule,
     }
     if fd_cells > 0:
         rng = np.random.default_rng(seed)
