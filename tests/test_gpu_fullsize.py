@@ -1,5 +1,5 @@
 """Full-size checks of the HIP path where the oracle is too slow: exact
-linearity.
+linearity, and the exact mass decay of the zeroth moment on a periodic grid.
 
 For fixed materials and filters the scheme is linear in the state and the
 source, and scaling by 2 commutes with rounding (the values here are far from
@@ -87,3 +87,24 @@ def test_lattice_p39_1000_linear(precision):
 def test_gauss3d_p7_200_linear():
     """3D P7, 200^3 periodic, anisotropic scattering: two steps in FP64."""
     check_linear(P.gauss_periodic_3d(n=200, n_mom=7, tfinal=0.75), "fp64", 2)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 2e-6)])
+def test_gauss3d_p7_200_mass_decay(precision, tol):
+    """3D P7, 200^3 periodic, sigma_a = 0.1, no source: the flux of the
+    zeroth moment sums to zero over the periodic grid, so its total decays by
+    exactly exp(-sigma_a*dt) per step (two half-step applications, each
+    multiplying it by 1 - dt/2*sigma_a*expm1div(-sigma_a*dt/2) =
+    exp(-sigma_a*dt/2)), to the rounding of the state and of the sums."""
+    s = HipSolver(I.starmap_init(P.gauss_periodic_3d(n=200, n_mom=7, tfinal=0.75)), precision=precision)
+    try:
+        dt = s.prepare()
+        m0 = s.observables()[0][0]
+        s.step(dt, 4)
+        m4 = s.observables()[0][0]
+        expect = m0 * np.exp(-0.1 * dt * 4)
+        print("mass", precision, m0, m4, expect, (m4 - expect) / expect, flush=True)
+        assert m0 > 0
+        assert abs(m4 - expect) <= tol * expect
+    finally:
+        s.close()
