@@ -190,6 +190,51 @@ def boxes_2d(n=50, n_mom=9, closure="P", tfinal=0.2):
     }
 
 
+def controlrod_2d(n=251, n_mom=3, closure="SP", tfinal=0.9, n_plot=46):
+    """starmap_2d_ex_controlrod.m:35-81: a control rod moves in and out of a
+    multiplying medium.  Effective absorption rod(x,y)*s(t), with s rising
+    linearly to 10 over [0,0.2], held to 0.4 and falling to 0 at 0.6;
+    effective scattering sigma_s + nu*sigma_f = 1 + 0.9*2; initial zeroth
+    moment 1e6; periodic [-1,1]^2.  sigma_a takes t, so the solver runs its
+    time-dependent branch (materials averaged over each step, the last step
+    ending on tfinal)."""
+    def rod(x, y):
+        return (((-0.5 <= x) & (x <= 0.2) & (-0.5 <= y) & (y <= 0)) |
+                ((-0.3 <= x) & (x <= 0) & (0 < y) & (y <= 0.6)) |
+                ((0 < x) & (x <= 0.5) & (0 < y) & (y < 0.3))).astype(np.float64)
+
+    def time_dependence(t):
+        T, dT, smax = 0.6, 0.2, 10.0
+        return (smax / dT * t * ((0 < t) & (t <= dT)) + smax * ((dT < t) & (t <= T - dT)) +
+                smax / dT * (T - t) * ((T - dT < t) & (t <= T)))
+
+    return {
+        "name": "Control Rod Test",
+        "closure": closure,
+        "n_mom": n_mom,
+        "sigma_a": lambda x, y, t: rod(x, y) * time_dependence(t),
+        "sigma_s0": lambda x, y: (1.0 + 0.9 * 2.0) * np.ones(np.shape(x)),
+        "ic": lambda x, y: 1e6 * np.ones(np.shape(x)),
+        "ax": [-1.0, 1.0, -1.0, 1.0],
+        "n": [n, n],
+        "bc": [0, 0],
+        "t_plot": list(np.linspace(0, tfinal, n_plot)),
+        "tfinal": tfinal,
+        "output": None,
+    }
+
+
+def exp_filter_moving(p, f_eff, speed=2.0):
+    """A filter with the argument list (par, dt, k, x, y, t), which the
+    solver re-evaluates every step (starmap_solver.m:284): the exponential
+    filter of order p whose strength varies in space and time,
+    exp(-dt * f_eff * (1 + 0.5 sin(speed t) + 0.1 x) * (k/N)^p)."""
+    def filterfunction(par, dt, k, x, y, t):
+        N = par["n_mom"]
+        return np.exp(-dt * f_eff * (1 + 0.5 * np.sin(speed * t) + 0.1 * x) * (k / N) ** p)
+    return filterfunction
+
+
 BATCH_SIDES = tuple(range(64, 513, 32))     # 64, 96, ..., 512
 BATCH_ORDERS = tuple(range(3, 40, 2))       # odd N = 3, 5, ..., 39
 

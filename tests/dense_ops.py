@@ -143,3 +143,96 @@ def fourier_step_matrix(Mx, My, Mz, xi, h, dt, s, E, F1=None, F2=None):
         return A
 
     return half(odd, 2) @ half(~odd, 2) @ half(~odd, 1) @ half(odd, 1), gt
+
+
+# ---------------------------------------------------------------------------
+# Affine one-step model with per-point coefficients
+# ---------------------------------------------------------------------------
+def flux_matrix(Mx, My, Mz, n, bc, h):
+    """The flux operator W = -(sum_d M_d (x) D_d) as one dense matrix over
+    the concatenated state (each component raveled column-major on its own
+    staggered grid), plus the component offsets and grid types."""
+    gt = two_colour(Mx, My, Mz)
+    ext = extents(gt, n, bc)
+    size = ext.prod(axis=1)
+    off = np.concatenate([[0], np.cumsum(size)])
+    N = off[-1]
+    W = np.zeros((N, N))
+    for j in range(Mz.shape[0]):
+        for d, M in enumerate((Mx, My, Mz)):
+            for k in np.nonzero(M[j])[0]:
+                mats = []
+                for a in range(3):
+                    if a == d:
+                        mats.append(_diff_1d(n[a], bc[a], gt[k, a], h[a]))
+                    else:
+                        mats.append(np.eye(ext[k, a]))
+                D = np.kron(mats[2], np.kron(mats[1], mats[0]))
+                W[off[j]:off[j + 1], off[k]:off[k + 1]] -= M[j, k] * D
+    return W, off, gt
+
+
+def staggered_points(gt_j, n, bc, lo, h):
+    """Coordinates (X, Y, Z), raveled column-major, of the points of a
+    component with grid types gt_j: cell centres lo+(i+1/2)h; faces lo+(i+1)h
+    (periodic, n of them) or lo+ih (extrapolation, n+1 of them)."""
+    axes = []
+    for a in range(3):
+        if gt_j[a] == 1:
+            axes.append(lo[a] + (np.arange(n[a]) + 0.5) * h[a])
+        elif bc[a] == 0:
+            axes.append(lo[a] + (np.arange(n[a]) + 1.0) * h[a])
+        else:
+            axes.append(lo[a] + np.arange(n[a] + 1) * h[a])
+    X, Y, Z = np.meshgrid(*axes, indexing="ij")
+    return X.ravel(order="F"), Y.ravel(order="F"), Z.ravel(order="F")
+
+
+def affine_step(W, off, odd, even, dt, s, E, Q=None, F1=None, F2=None, Ffull=None):
+    """One time step as the affine map u -> A u + b.
+
+    W: flux matrix (flux_matrix); s, E: per component, the decay rate and
+    expm1div(-s dt/2) at each of its points (arrays); Q: per component the
+    source at each point, or None; F1, F2: substep filter factors per
+    component and point (slot 1: first odd and first even half-step; slot 2:
+    second even and last odd), or None; Ffull: filter factors applied to
+    every component but the zeroth after the step, or None.
+
+    Each half-step on its targets j is u_j <- f_j (u_j + dt/2 E_j (W_j u + Q_j
+    - s_j u_j)), the exact solution of u' = W u + Q - s u over dt/2 with W u
+    and Q frozen, times the filter factor f_j (1 for j = 0)."""
+    N = W.shape[0]
+    I = np.eye(N)
+
+    def half(targets, slot):
+        A = np.eye(N)
+        b = np.zeros(N)
+        for j in targets:
+            sl = slice(off[j], off[j + 1])
+            e = np.broadcast_to(np.asarray(E[j], dtype=float), (sl.stop - sl.start,))
+            sig = np.broadcast_to(np.asarray(s[j], dtype=float), e.shape)
+            rows = I[sl] + (dt / 2) * e[:, None] * (W[sl] - sig[:, None] * I[sl])
+            bj = (dt / 2) * e * (np.asarray(Q[j], dtype=float) if Q is not None and Q[j] is not None
+                                 else 0.0)
+            if F1 is not None and j != 0:
+                f = np.broadcast_to(np.asarray(F1[j] if slot == 1 else F2[j], dtype=float), e.shape)
+                rows = f[:, None] * rows
+                bj = f * bj
+            A[sl] = rows
+            b[sl] = bj
+        return A, b
+
+    def then(second, first):
+        (A2, b2), (A1, b1) = second, first
+        return A2 @ A1, A2 @ b1 + b2
+
+    step = half(odd, 1)
+    step = then(half(even, 1), step)
+    step = then(half(even, 2), step)
+    step = then(half(odd, 2), step)
+    if Ffull is not None:
+        d = np.ones(N)
+        for j in range(1, len(off) - 1):
+            d[off[j]:off[j + 1]] = Ffull[j]
+        step = (d[:, None] * step[0], d * step[1])
+    return step
