@@ -36,7 +36,9 @@ extern "C" {
 #define SMP_E_HIP       2   /* a HIP runtime call failed                         */
 #define SMP_E_OOM       3   /* device or host allocation failed                  */
 #define SMP_E_STATE     4   /* call not valid in the handle's current state      */
-#define SMP_E_UNSTABLE  5   /* non-finite values detected in the state           */
+#define SMP_E_UNSTABLE  5   /* non-finite values detected in the state (smp_step,
+                               smp_interp, smp_check_finite); the reference
+                               carries on with them                            */
 
 /* state precision */
 #define SMP_FP64 0
@@ -131,13 +133,24 @@ int smp_set_coeff(smp_problem *h, int32_t kind, const int32_t gt[3], int32_t ord
 
 /* Advance n_steps full time steps of size dt: the three Strang substeps
  * [1 2 1] and the optional full filter (starmap_solver.m:307-421).  Kernels
- * are queued on the handle's stream; the call returns after they finish. */
+ * are queued on the handle's stream; the call returns after they finish.
+ * Once at least 64 steps (SMP_CHECK_EVERY in the environment at smp_create;
+ * 0 turns it off) have run since the last scan, the call ends with one pass
+ * over the state that returns SMP_E_UNSTABLE if any value is NaN or
+ * infinite -- the monitor of a time step beyond the stability limit, which
+ * the reference runs through silently. */
 int smp_step(smp_problem *h, double dt, int64_t n_steps);
+
+/* Scan the whole state once: *nonfinite = 1 and SMP_E_UNSTABLE if any value
+ * is NaN or infinite, else 0 and SMP_OK.  The Python mirror calls it after
+ * the last step of a solve.  No counterpart in the reference. */
+int smp_check_finite(smp_problem *h, int32_t *nonfinite);
 
 /* Output accumulation (starmap_solver.m:308, :425-428):
  * smp_snapshot keeps U0 = U(comps) on the device; smp_interp writes
  * (1-lambda)*U0 + lambda*U(comps) to the host, components concatenated in the
- * reference layout (count = total elements). */
+ * reference layout (count = total elements), and returns SMP_E_UNSTABLE
+ * (after writing) if a written value is NaN or infinite. */
 int smp_snapshot(smp_problem *h, const int32_t *comps, int32_t n_comps);
 int smp_interp(smp_problem *h, double lambda, double *out, int64_t count);
 

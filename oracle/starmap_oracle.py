@@ -411,7 +411,9 @@ def starmap_solver(par, verbose=True, return_state=False):
     desc.f_position = {"": 0, "substep": 1, "full": 2}[fpos]
 
     # time loop, :162-475
-    t_plot = list(np.atleast_1d(par["t_plot"]).astype(float)) + [np.inf]
+    # :162, par.t_plot = [par.t_plot inf]; the loop reads t_plot, tfinal,
+    # cn and mom_output from par, which an output callback may replace
+    par["t_plot"] = list(np.atleast_1d(par["t_plot"]).astype(float)) + [np.inf]
     plot_count = 0
     cpu = np.zeros(3)
     t = 0.0
@@ -487,14 +489,15 @@ def starmap_solver(par, verbose=True, return_state=False):
                     filterTerm[s + (2,)] = capargs(par["filterfunction"], par, dt, s[3],
                                                    *grids[g], t)
         cpu[1] += time.perf_counter() - t0c
+        mom_output = [int(j) - 1 for j in np.atleast_1d(par["mom_output"])]
         U0 = [U[j].copy() for j in mom_output]
         t0c = time.perf_counter()
         _run_step(desc, dt, U, Q, gkey, mom_order, f_mom_order, sA1, sT1, EA, ET,
                   filterTerm, fpos)
         cpu[0] += time.perf_counter() - t0c
         t0c = time.perf_counter()
-        while t >= t_plot[plot_count] - 1e-14:
-            lam_ = (t_plot[plot_count] - t + dt) / dt
+        while t >= par["t_plot"][plot_count] - 1e-14:
+            lam_ = (par["t_plot"][plot_count] - t + dt) / dt
             Uplot = [(1 - lam_) * a + lam_ * b for a, b in zip(U0, [U[j] for j in mom_output])]
             if "output" in par and callable(par["output"]):
                 r = _call_output(par, Uplot, mom_output, gt, coords, plot_count + 1)
@@ -503,9 +506,13 @@ def starmap_solver(par, verbose=True, return_state=False):
             plot_count += 1
         cpu[2] += time.perf_counter() - t0c
     if verbose:
+        # :477-482
         print("%s with %s%s%d\n %d moments, %dx%dx%d grid, %0.0f time steps"
               % (par["name"], par["closure_mod"], par["closure"], par["n_mom"], n_sys,
                  nn[0], nn[1], nn[2], step_count))
+        pct = cpu / (cpu.sum() or 1.0) * 1e2
+        print("CPU-times\n advection:%12.2fs%5.0f%%\n absorb/scatter:%7.2fs%5.0f%%\n"
+              " plotting:%13.2fs%5.0f%%" % (cpu[0], pct[0], cpu[1], pct[1], cpu[2], pct[2]))
     out = _final_output(par, U, gt, coords)
     par["_oracle"] = {"step_count": step_count, "dt": dt, "t": t, "cputime": cpu,
                       "gt": gt, "dt_hyp": dt_hyp}

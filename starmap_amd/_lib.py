@@ -27,7 +27,8 @@ EXPORTS = ["smp_last_error", "smp_version", "smp_create", "smp_destroy", "smp_gr
            "smp_set_schedule", "smp_get_schedule", "smp_checkpoint_save", "smp_checkpoint_load",
            "smp_checkpoint_free", "smp_batch_create", "smp_batch_destroy", "smp_batch_run",
            "smp_batch_timing", "smp_adjoint_reset", "smp_adjoint_seed", "smp_adjoint",
-           "smp_adjoint_get", "smp_adjoint_timing", "smp_moment_norms", "smp_probe"]
+           "smp_adjoint_get", "smp_adjoint_timing", "smp_moment_norms", "smp_probe",
+           "smp_check_finite"]
 
 
 class SmpError(RuntimeError):
@@ -103,6 +104,7 @@ def lib():
         "smp_adjoint_timing": (ctypes.c_int, [vp, vp, vp]),
         "smp_moment_norms": (ctypes.c_int, [vp, vp, vp]),
         "smp_probe": (ctypes.c_int, [vp, vp, vp, vp]),
+        "smp_check_finite": (ctypes.c_int, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -112,9 +114,15 @@ def lib():
     return L
 
 
+class SmpUnstable(SmpError):
+    """SMP_E_UNSTABLE: NaN or infinite values in the state, the sign of a
+    time step beyond the stability limit (par.cn too large)."""
+
+
 def check(rc):
     if rc != SMP_OK:
-        raise SmpError(rc, lib().smp_last_error().decode())
+        cls = SmpUnstable if rc == SMP_E_UNSTABLE else SmpError
+        raise cls(rc, lib().smp_last_error().decode())
 
 
 def ptr(a):

@@ -34,12 +34,16 @@ int fail(int code, const std::string &msg) {
     return code;
 }
 
+// On failure the runtime's last-error state is cleared as well, so that a
+// later launch check (hipGetLastError) does not report this error again.
 #define HIP_TRY(expr)                                                                     \
     do {                                                                                  \
         hipError_t e_ = (expr);                                                           \
-        if (e_ != hipSuccess)                                                             \
+        if (e_ != hipSuccess) {                                                           \
+            (void)hipGetLastError();                                                      \
             return fail(e_ == hipErrorOutOfMemory ? SMP_E_OOM : SMP_E_HIP,                \
                         std::string(#expr) + ": " + hipGetErrorString(e_));               \
+        }                                                                                 \
     } while (0)
 
 #define REQUIRE_DEVICE(h)                                                                 \
@@ -100,7 +104,9 @@ struct smp_problem {
     void *snap = nullptr;
     void *interp_buf = nullptr;
     int64_t interp_cap = 0;
-    void *red_buf = nullptr;
+    void *red_buf = nullptr;      // reduction partials, results and the non-finite flag
+    int check_every = 64;         // steps between non-finite scans in smp_step (0: none)
+    int64_t steps_unchecked = 0;
     void *norm_buf = nullptr;   // partials and results of smp_moment_norms
     void *sink = nullptr;         // 512 elements: stores of lanes without a target cell
     std::vector<hipEvent_t> events;
@@ -133,6 +139,7 @@ struct smp_problem {
     int dalloc(void **p, size_t bytes) {
         hipError_t e = hipMalloc(p, bytes ? bytes : 1);
         if (e != hipSuccess) {
+            (void)hipGetLastError();
             *p = nullptr;
             return fail(e == hipErrorOutOfMemory ? SMP_E_OOM : SMP_E_HIP,
                         std::string("hipMalloc(") + std::to_string(bytes) + "): " +
@@ -290,6 +297,25 @@ int make_plan(smp_problem *h, int kind, int ncf_reserve) {
     return h->dalloc(&h->plan_dev[kind], std::max<size_t>(1, plan.groups.size()) * smp::BLOB_SZ);
 }
 
+// Scan the whole state for NaN or infinite values (one pass over the
+// state; the flag lives after the reduction results in red_buf and is
+// cleared by a fill kernel).
+int scan_state(smp_problem *h, bool *bad) {
+    int *flag = (int *)((double *)h->red_buf + 3 * 1024 + 3);
+    const int64_t count = h->slab * h->geo.n_sys;
+    HIP_TRY(smp::launch_fill(flag, 1, 0, h->stream));
+    if (h->precision == SMP_FP64)
+        HIP_TRY(smp::launch_finite<double>((const double *)h->U, count, flag, h->stream));
+    else
+        HIP_TRY(smp::launch_finite<float>((const float *)h->U, count, flag, h->stream));
+    int v = 0;
+    HIP_TRY(hipMemcpyAsync(&v, flag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->steps_unchecked = 0;
+    *bad = v != 0;
+    return SMP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -385,6 +411,7 @@ int smp_create(const smp_desc *d, smp_problem **out) {
     if (cw != 4 && cw != 8) return fail(SMP_E_INVALID, "cell_waves must be 4 or 8");
     h->cell_waves = cw;
     if (want_blocks < 1 || want_blocks > 8) return fail(SMP_E_INVALID, "blocks_per_cu must be 1..8");
+    if (const char *env = std::getenv("SMP_CHECK_EVERY")) h->check_every = std::max(0, std::atoi(env));
     h->blocks_per_cu = want_blocks;
     for (int kind = 0; kind < 2; ++kind) {
         int rc = make_plan(h.get(), kind, 0);
@@ -398,7 +425,7 @@ int smp_create(const smp_desc *d, smp_problem **out) {
     int rc = h->dalloc(&h->U, (size_t)h->slab * h->elem * d->n_sys);
     if (rc) return rc;
     HIP_TRY(hipMemsetAsync(h->U, 0, (size_t)h->slab * h->elem * d->n_sys, h->stream));
-    rc = h->dalloc(&h->red_buf, sizeof(double) * (3 * 1024 + 3));
+    rc = h->dalloc(&h->red_buf, sizeof(double) * (3 * 1024 + 4));
     if (rc) return rc;
     rc = h->dalloc(&h->sink, sizeof(double) * 512);
     if (rc) return rc;
@@ -1315,6 +1342,9 @@ int batch_run(smp_batch *b, const double *dt, const int64_t *ns, int use_graph) 
             if (rc) return rc;
         }
     }
+    // the decay kernels ran on each problem's own stream: finish them before
+    // the batch's stream reads the E fields
+    for (int i = 0; i < n; ++i) HIP_TRY(hipStreamSynchronize(b->hs[i]->stream));
     b->last_ms = b->last_kernel_ms = b->last_bytes = 0;
     b->last_launches = b->last_mcu = 0;
     if (max_n == 0) return SMP_OK;
@@ -1525,8 +1555,33 @@ int smp_step(smp_problem *h, double dt, int64_t n_steps) {
     REQUIRE_DEVICE(h);
     HIP_TRY(hipSetDevice(h->device));
     if (n_steps == 0) return SMP_OK;
-    return h->precision == SMP_FP64 ? run_steps<double>(h, dt, n_steps)
-                                    : run_steps<float>(h, dt, n_steps);
+    int rc = h->precision == SMP_FP64 ? run_steps<double>(h, dt, n_steps)
+                                      : run_steps<float>(h, dt, n_steps);
+    if (rc) return rc;
+    h->steps_unchecked += n_steps;
+    if (h->check_every > 0 && h->steps_unchecked >= h->check_every) {
+        bool bad = false;
+        rc = scan_state(h, &bad);
+        if (rc) return rc;
+        if (bad)
+            return fail(SMP_E_UNSTABLE, "non-finite values in the state after " +
+                                            std::to_string(n_steps) + " steps of dt = " +
+                                            std::to_string(dt) +
+                                            " (unstable time step: is par.cn too large?)");
+    }
+    return SMP_OK;
+}
+
+int smp_check_finite(smp_problem *h, int32_t *nonfinite) {
+    if (!h) return fail(SMP_E_INVALID, "null handle");
+    REQUIRE_DEVICE(h);
+    HIP_TRY(hipSetDevice(h->device));
+    bool bad = false;
+    int rc = scan_state(h, &bad);
+    if (rc) return rc;
+    if (nonfinite) *nonfinite = bad ? 1 : 0;
+    if (bad) return fail(SMP_E_UNSTABLE, "non-finite values in the state");
+    return SMP_OK;
 }
 
 int smp_snapshot(smp_problem *h, const int32_t *comps, int32_t n_comps) {
@@ -1588,6 +1643,9 @@ int smp_interp(smp_problem *h, double lambda, double *out, int64_t count) {
         h->unpack<double>(gt, tmp.data(), out + pos);
         pos += h->numel(j);
     }
+    for (int64_t i = 0; i < count; ++i)
+        if (!std::isfinite(out[i]))
+            return fail(SMP_E_UNSTABLE, "non-finite values in the interpolated output");
     return SMP_OK;
 }
 

@@ -124,6 +124,11 @@ hipError_t launch_reduce(const T *U, const ReduceGeom &g, double *part, int bloc
 
 hipError_t launch_reduce_final(const double *part, int blocks, double *out3, hipStream_t s);
 
+// Sets *flag to 1 if any of the count elements of U is NaN or infinite
+// (the flag is not cleared here).
+template <typename T>
+hipError_t launch_finite(const T *U, int64_t count, int *flag, hipStream_t s);
+
 // Moment norms of one component: sum of squares and, per kernel axis, of
 // squared differences of neighbours in the reference's output order
 // (rot[e]: periodic face axis, stored index (f+1) mod n).

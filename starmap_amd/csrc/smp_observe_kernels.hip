@@ -7,6 +7,7 @@
 // starmap_2d_ex_linesource_filter_reconstruction.m:118-121).
 #include "smp_kernels.h"
 
+#include <algorithm>
 #include <cmath>
 
 namespace smp {
@@ -90,6 +91,30 @@ __global__ void probe_kernel(const T *U, int64_t slab, const ProbeComp *pc, int 
     out[j] = c.nan ? NAN : s;
 }
 
+// flag = 1 if any of U[0..count) is NaN or infinite: a grid-stride scan of
+// the whole state (padding elements hold zeros), one atomic OR per wave that
+// saw such a value.  The flag is cleared by the caller.
+template <typename T>
+__global__ __launch_bounds__(256) void finite_kernel(const T *U, int64_t count, int *flag) {
+    bool bad = false;
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    for (; i + 3 * step < count; i += 4 * step) {
+        const T a = U[i], b = U[i + step], c = U[i + 2 * step], d = U[i + 3 * step];
+        bad |= !(isfinite(a) && isfinite(b) && isfinite(c) && isfinite(d));
+    }
+    for (; i < count; i += step) bad |= !isfinite(U[i]);
+    if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
+template <typename T>
+hipError_t launch_finite(const T *U, int64_t count, int *flag, hipStream_t s) {
+    int64_t blocks = (count + 4 * 256 - 1) / (4 * 256);
+    blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, 8192));
+    hipLaunchKernelGGL(finite_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, s, U, count, flag);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_norm(const T *U, const NormGeom &g, double *part, int blocks, double *out4,
                        hipStream_t s) {
@@ -107,6 +132,7 @@ hipError_t launch_probe(const T *U, int64_t slab, const ProbeComp *pc, int n, do
 }
 
 #define SMP_OBS_INSTANTIATE(T)                                                              \
+    template hipError_t launch_finite<T>(const T *, int64_t, int *, hipStream_t);             \
     template hipError_t launch_norm<T>(const T *, const NormGeom &, double *, int, double *,  \
                                        hipStream_t);                                        \
     template hipError_t launch_probe<T>(const T *, int64_t, const ProbeComp *, int, double *, \

@@ -367,15 +367,17 @@ class HipSolver:
             c_even = [j for j in range(self.n_sys) if self.gkey[j] in even_g]
             c_odd = [j for j in range(self.n_sys) if self.gkey[j] not in even_g]
         L = _lib.lib()
-        mom_output = [int(j) - 1 for j in np.atleast_1d(par["mom_output"])]
         td_mat = any(self.time_dependent)
         td_src = len(par["source_ind"]) > 0 and nargin(par["source"]) > 3
         td_filt = self.flag_filter and nargin(par["filterfunction"]) > 6
-        t_plot = [float(v) for v in np.atleast_1d(par["t_plot"])] + [np.inf]
+        # par.t_plot = [par.t_plot inf] (:162): callbacks see it, and a
+        # callback that returns par may change t_plot, tfinal, cn and
+        # mom_output for the rest of the loop, which reads them from par
+        par = dict(par, t_plot=[float(v) for v in np.atleast_1d(par["t_plot"])] + [np.inf])
+        self.par = par
         pc = 0
         cpu = np.zeros(3)
         t, dt, step_count, pending = 0.0, 0.0, 0, 0
-        tfinal, cn = par["tfinal"], par["cn"]
         sA = sT = None
         self.initial_state()
 
@@ -387,9 +389,10 @@ class HipSolver:
                 cpu[0] += time.perf_counter() - c0
                 pending = 0
 
-        while t < tfinal:
+        while t < par["tfinal"]:
             c0 = time.perf_counter()
-            tleft = (tfinal - t) / cn
+            cn = par["cn"]
+            tleft = (par["tfinal"] - t) / cn
             if timestep > 0 and (t == 0 or td_mat):
                 flush()
                 # materials and dt, :168-266 with the stability search
@@ -445,15 +448,16 @@ class HipSolver:
                 flush()
                 self.upload_filter(dt, t)
             cpu[1] += time.perf_counter() - c0
-            if t >= t_plot[pc] - 1e-14:
+            if t >= par["t_plot"][pc] - 1e-14:
                 flush()
+                mom_output = [int(j) - 1 for j in np.atleast_1d(par["mom_output"])]
                 cs = np.array(mom_output, dtype=np.int32)
                 _lib.check(L.smp_snapshot(self.handle, cs.ctypes.data, len(cs)))
                 pending = 1
                 flush()
                 c0 = time.perf_counter()
-                while t >= t_plot[pc] - 1e-14:
-                    lam = (t_plot[pc] - t + dt) / dt
+                while t >= par["t_plot"][pc] - 1e-14:
+                    lam = (par["t_plot"][pc] - t + dt) / dt
                     total = sum(int(np.prod(self.shape(j))) for j in mom_output)
                     buf = np.empty(total, dtype=np.float64)
                     _lib.check(L.smp_interp(self.handle, lam, buf.ctypes.data, buf.size))
@@ -469,6 +473,9 @@ class HipSolver:
         flush()
         self.step_count, self.dt, self.t = step_count, dt, t
         self.cputime = cpu
+        # one scan of the final state: SmpUnstable if a value is not finite
+        bad = ctypes.c_int32()
+        _lib.check(L.smp_check_finite(self.handle, ctypes.byref(bad)))
         if verbose:
             print("%s with %s%s%d\n %d moments, %dx%dx%d grid, %0.0f time steps"
                   % (par["name"], par["closure_mod"], par["closure"], par["n_mom"], self.n_sys,

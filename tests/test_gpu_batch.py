@@ -19,6 +19,7 @@ from starmap_amd import init as I
 from starmap_amd import problems as P
 from starmap_amd.batch import BatchSolver, count_steps
 from starmap_amd.solver import HipSolver
+from tests.fp32_bounds import check_fp32_bound
 
 pytestmark = pytest.mark.gpu
 
@@ -76,8 +77,9 @@ def test_batch_bitwise_equal_single_handles(precision, cw, use_graph):
         b.close()
 
 
-def test_fp32_batch_within_1e5_of_fp64_oracle():
-    pars = small_sweep(n=4, orders=(7, 3, 5), tfinal=0.2)
+@pytest.mark.parametrize("n,orders", [(4, (7, 3, 5)), (2, (31, 39))])
+def test_fp32_batch_within_1e5_of_fp64_oracle(n, orders):
+    pars = small_sweep(n=n, orders=orders, tfinal=0.2)
     b = BatchSolver(pars, precision="fp32")
     try:
         b.run()
@@ -85,12 +87,9 @@ def test_fp32_batch_within_1e5_of_fp64_oracle():
             print("oracle", i, p["n"][1], p["n_mom"], flush=True)
             ref, rpar = so.starmap_solver(dict(p), verbose=False, return_state=True)
             assert rpar["_oracle"]["step_count"] == b.n_steps[i]
-            norms = np.array([np.linalg.norm(r["U"]) for r in ref])
-            big = norms.max()
-            for j, r in enumerate(ref):
-                d = np.linalg.norm(s.get_state(j).reshape(np.shape(r["U"]), order="F") - r["U"])
-                e = d / norms[j] if norms[j] > 1e-8 * big else d / big
-                assert e <= 1e-5, (i, j, e)
+            got = [s.get_state(j).reshape(np.shape(r["U"]), order="F") for j, r in enumerate(ref)]
+            e = check_fp32_bound(got, [r["U"] for r in ref])
+            print("  fp32 errors", e, flush=True)
     finally:
         b.close()
 

@@ -198,3 +198,43 @@ def test_output_interpolation_between_steps():
     lam = (tp - 3 * dt + dt) / dt
     want = (1 - lam) * outs[0] + lam * outs[1]
     assert np.max(np.abs(got[1] - want)) < 1e-13 * np.max(np.abs(want))
+
+
+def test_callback_returning_par_changes_the_run():
+    """An output callback that returns par replaces it for the rest of the
+    loop (starmap_solver.m:441-469): here it halves tfinal at the first
+    output, so the run stops early and the second output time is never
+    reached.  Callbacks see par.t_plot with inf appended (:162)."""
+    seen = []
+
+    def output(par, x, y, U, step):
+        seen.append((step, list(par["t_plot"])))
+        if step == 1:
+            return dict(par, tfinal=par["tfinal"] / 2)
+        return None
+
+    base = P.gauss_periodic_2d(n=20, n_mom=3, std=0.2, tfinal=0.2, mom_output=1)
+    par = so.starmap_init(dict(base, t_plot=[0.05, 0.2], output=output))
+    out, p2 = so.starmap_solver(par, verbose=False, return_state=True)
+    assert [s for s, _ in seen] == [1]
+    assert seen[0][1] == [0.05, 0.2, np.inf]
+    short, p3 = so.starmap_solver(so.starmap_init(dict(base, tfinal=0.1, t_plot=[])),
+                                  verbose=False, return_state=True)
+    assert p2["_oracle"]["step_count"] == p3["_oracle"]["step_count"]
+    assert np.array_equal(out[0]["U"], short[0]["U"])
+
+
+def test_summary_print_format(capsys):
+    """The overview and CPU-time lines of starmap_solver.m:477-482."""
+    par = so.starmap_init(P.gauss_periodic_2d(n=12, n_mom=3, tfinal=0.05))
+    so.starmap_solver(par, verbose=True)
+    lines = capsys.readouterr().out.splitlines()
+    assert lines[0] == "Gaussian periodic 2D with P3"
+    assert lines[1] == " 10 moments, 1x12x12 grid, 1 time steps"
+    assert lines[2] == "CPU-times"
+    import re
+    pat = [r" advection: {0,11}\d+\.\d\ds {0,4}\d+%", r" absorb/scatter: {0,6}\d+\.\d\ds {0,4}\d+%",
+           r" plotting: {0,12}\d+\.\d\ds {0,4}\d+%"]
+    for ln, p in zip(lines[3:6], pat):
+        assert re.fullmatch(p, ln), ln
+    assert len(lines[3]) == len(" advection:") + 13 + 6
