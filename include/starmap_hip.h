@@ -278,6 +278,23 @@ int smp_moment_norms(smp_problem *h, double *sumsq, double *diffsq);
  * par.ax(1:2:end)); out has n_sys entries. */
 int smp_probe(smp_problem *h, const double *lo, const double *point, double *out);
 
+/* ---- allocation-free variant (no counterpart in the reference) --------- */
+
+/* Largest n_sys smp_probe_noalloc accepts. */
+#define SMP_PROBE_NOALLOC_MAX_COMPS 4096
+
+/* smp_probe without any device allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving or
+ * monitoring loops, harnesses that forbid hipMalloc): the per-call component
+ * table and values live in one fixed device array of the library (about
+ * 590 KB per GPU, held until the process ends) instead of the two buffers
+ * smp_probe allocates and frees on every call.  Same arguments, results and
+ * errors as smp_probe; SMP_E_INVALID for a problem with more than
+ * SMP_PROBE_NOALLOC_MAX_COMPS components.  Every call on one GPU shares that
+ * array, so calls that overlap on the same GPU (from several threads) race;
+ * the call is synchronous, so calls in sequence do not. */
+int smp_probe_noalloc(smp_problem *h, const double *lo, const double *point, double *out);
+
 #ifdef __cplusplus
 }
 #endif

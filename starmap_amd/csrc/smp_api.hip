@@ -17,6 +17,7 @@
 #include <map>
 #include <set>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -316,6 +317,117 @@ int scan_state(smp_problem *h, bool *bad) {
     return SMP_OK;
 }
 
+// Per component, the stored indices and weights of the 2^ndim grid points
+// around `point` on the component's own staggered grid (NaN outside).
+void probe_table(const smp_problem *h, const double *lo, const double *point,
+                 std::vector<smp::ProbeComp> &pc) {
+    const smp::Geometry &g = h->geo;
+    const int64_t stride[3] = {1, h->pitch0, (int64_t)h->pitch0 * h->E[1]};
+    pc.assign(g.n_sys, smp::ProbeComp());
+    for (int j = 0; j < g.n_sys; ++j) {
+        smp::ProbeComp &c = pc[j];
+        std::memset(&c, 0, sizeof(c));
+        // per kernel axis: the two stored indices and weights of the point
+        int64_t s0[3] = {0, 0, 0}, s1[3] = {0, 0, 0};
+        double t[3] = {0, 0, 0};
+        for (int e = 0; e < g.ndim; ++e) {
+            const int ax = g.axis[e];
+            const bool face = g.gt[3 * j + ax] == 2;
+            const int n = g.n[ax], bc = g.bc[ax];
+            const double hh = g.h[ax];
+            // the reference's grid vector of this grid type (starmap_solver.m:113-118)
+            const double first = lo[ax] + (face ? (bc ? 0.0 : hh) : hh / 2);
+            const int len = face ? n + bc : n;
+            double f = (point[ax] - first) / hh;
+            // the grid vector is first + k*h here and MATLAB's colon there:
+            // points within 1e-9 cells of either end are on it
+            if (f < 0 && f > -1e-9) f = 0;
+            if (f > len - 1 && f < len - 1 + 1e-9) f = len - 1;
+            if (!(f >= 0) || f > len - 1) {
+                c.nan = 1;
+                break;
+            }
+            int f0 = (int)std::floor(f);
+            if (f0 > len - 2) f0 = len > 1 ? len - 2 : 0;
+            t[e] = len > 1 ? f - f0 : 0.0;
+            const int f1 = len > 1 ? f0 + 1 : f0;
+            auto stored = [&](int fi) { return face && bc == 0 ? (fi + 1) % n : fi; };
+            s0[e] = stored(f0);
+            s1[e] = stored(f1);
+        }
+        if (c.nan) continue;
+        const int corners = 1 << g.ndim;
+        for (int k = 0; k < corners; ++k) {
+            double w = 1;
+            int64_t off = 0;
+            for (int e = 0; e < g.ndim; ++e) {
+                const bool hi = (k >> e) & 1;
+                w *= hi ? t[e] : 1 - t[e];
+                off += (hi ? s1[e] : s0[e]) * stride[e];
+            }
+            c.off[k] = off;
+            c.w[k] = w;
+        }
+        c.n = corners;
+    }
+}
+
+// Upload the table to dpc, interpolate into dout and copy the values out.
+int probe_run(smp_problem *h, const std::vector<smp::ProbeComp> &pc, smp::ProbeComp *dpc,
+              double *dout, double *out) {
+    const smp::Geometry &g = h->geo;
+    HIP_TRY(hipMemcpyAsync(dpc, pc.data(), sizeof(smp::ProbeComp) * pc.size(), hipMemcpyHostToDevice,
+                           h->stream));
+    if (h->precision == SMP_FP64)
+        HIP_TRY(smp::launch_probe<double>((const double *)h->U, h->slab, dpc, g.n_sys, dout, h->stream));
+    else
+        HIP_TRY(smp::launch_probe<float>((const float *)h->U, h->slab, dpc, g.n_sys, dout, h->stream));
+    HIP_TRY(hipMemcpyAsync(out, dout, sizeof(double) * pc.size(), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return SMP_OK;
+}
+
+// The fixed device array of smp_probe_noalloc: the component table and the
+// values of SMP_PROBE_NOALLOC_MAX_COMPS components (about 590 KB on every
+// GPU the library runs on, held until the process ends).  Host code never
+// uses the symbol itself, only the address hipGetSymbolAddress returns for
+// the current device, resolved once per device when a handle is created.
+static __device__ __align__(256) unsigned char
+    scratch[SMP_PROBE_NOALLOC_MAX_COMPS * (sizeof(smp::ProbeComp) + sizeof(double))];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char *g_scratch_ptr[kMaxDevices] = {};
+
+// Resolve the array's address on `device` (which must be current).
+int resolve_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(SMP_E_INVALID, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return SMP_OK;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        return fail(SMP_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
+    g_scratch_ptr[device] = (char *)ptr;
+    return SMP_OK;
+}
+
+int probe_scratch(int device, char **base) {
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (device >= 0 && device < kMaxDevices && g_scratch_ptr[device]) {
+            *base = g_scratch_ptr[device];
+            return SMP_OK;
+        }
+    }
+    int rc = resolve_scratch(device);
+    if (rc) return rc;
+    *base = g_scratch_ptr[device];
+    return SMP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -368,6 +480,9 @@ int smp_create(const smp_desc *d, smp_problem **out) {
         if (d->device >= ndev) return fail(SMP_E_INVALID, "no such HIP device");
         HIP_TRY(hipSetDevice(d->device));
         HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        // address of smp_probe_noalloc's fixed array on this device
+        int rc = resolve_scratch(d->device);
+        if (rc) return rc;
     }
 
     // storage geometry
@@ -1739,75 +1854,33 @@ int smp_probe(smp_problem *h, const double *lo, const double *point, double *out
     if (!h || !lo || !point || !out) return fail(SMP_E_INVALID, "null argument");
     REQUIRE_DEVICE(h);
     HIP_TRY(hipSetDevice(h->device));
-    const smp::Geometry &g = h->geo;
-    const int64_t stride[3] = {1, h->pitch0, (int64_t)h->pitch0 * h->E[1]};
-    std::vector<smp::ProbeComp> pc(g.n_sys);
-    for (int j = 0; j < g.n_sys; ++j) {
-        smp::ProbeComp &c = pc[j];
-        std::memset(&c, 0, sizeof(c));
-        // per kernel axis: the two stored indices and weights of the point
-        int64_t s0[3] = {0, 0, 0}, s1[3] = {0, 0, 0};
-        double t[3] = {0, 0, 0};
-        for (int e = 0; e < g.ndim; ++e) {
-            const int ax = g.axis[e];
-            const bool face = g.gt[3 * j + ax] == 2;
-            const int n = g.n[ax], bc = g.bc[ax];
-            const double hh = g.h[ax];
-            // the reference's grid vector of this grid type (starmap_solver.m:113-118)
-            const double first = lo[ax] + (face ? (bc ? 0.0 : hh) : hh / 2);
-            const int len = face ? n + bc : n;
-            double f = (point[ax] - first) / hh;
-            // the grid vector is first + k*h here and MATLAB's colon there:
-            // points within 1e-9 cells of either end are on it
-            if (f < 0 && f > -1e-9) f = 0;
-            if (f > len - 1 && f < len - 1 + 1e-9) f = len - 1;
-            if (!(f >= 0) || f > len - 1) {
-                c.nan = 1;
-                break;
-            }
-            int f0 = (int)std::floor(f);
-            if (f0 > len - 2) f0 = len > 1 ? len - 2 : 0;
-            t[e] = len > 1 ? f - f0 : 0.0;
-            const int f1 = len > 1 ? f0 + 1 : f0;
-            auto stored = [&](int fi) { return face && bc == 0 ? (fi + 1) % n : fi; };
-            s0[e] = stored(f0);
-            s1[e] = stored(f1);
-        }
-        if (c.nan) continue;
-        const int corners = 1 << g.ndim;
-        for (int k = 0; k < corners; ++k) {
-            double w = 1;
-            int64_t off = 0;
-            for (int e = 0; e < g.ndim; ++e) {
-                const bool hi = (k >> e) & 1;
-                w *= hi ? t[e] : 1 - t[e];
-                off += (hi ? s1[e] : s0[e]) * stride[e];
-            }
-            c.off[k] = off;
-            c.w[k] = w;
-        }
-        c.n = corners;
-    }
+    std::vector<smp::ProbeComp> pc;
+    probe_table(h, lo, point, pc);
     void *dpc = nullptr, *dout = nullptr;
     int rc = h->dalloc(&dpc, sizeof(smp::ProbeComp) * pc.size());
     if (!rc) rc = h->dalloc(&dout, sizeof(double) * pc.size());
-    if (rc) {
-        h->dfree(dpc);
-        return rc;
-    }
-    HIP_TRY(hipMemcpyAsync(dpc, pc.data(), sizeof(smp::ProbeComp) * pc.size(), hipMemcpyHostToDevice,
-                           h->stream));
-    if (h->precision == SMP_FP64)
-        HIP_TRY(smp::launch_probe<double>((const double *)h->U, h->slab, (const smp::ProbeComp *)dpc,
-                                          g.n_sys, (double *)dout, h->stream));
-    else
-        HIP_TRY(smp::launch_probe<float>((const float *)h->U, h->slab, (const smp::ProbeComp *)dpc,
-                                         g.n_sys, (double *)dout, h->stream));
-    HIP_TRY(hipMemcpyAsync(out, dout, sizeof(double) * pc.size(), hipMemcpyDeviceToHost, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (!rc) rc = probe_run(h, pc, (smp::ProbeComp *)dpc, (double *)dout, out);
     h->dfree(dpc);
     h->dfree(dout);
-    return SMP_OK;
+    return rc;
+}
+
+int smp_probe_noalloc(smp_problem *h, const double *lo, const double *point, double *out) {
+    if (!h || !lo || !point || !out) return fail(SMP_E_INVALID, "null argument");
+    if (h->geo.n_sys > SMP_PROBE_NOALLOC_MAX_COMPS)
+        return fail(SMP_E_INVALID, "smp_probe_noalloc takes at most " +
+                                       std::to_string(SMP_PROBE_NOALLOC_MAX_COMPS) +
+                                       " components (this problem has " +
+                                       std::to_string(h->geo.n_sys) + ")");
+    REQUIRE_DEVICE(h);
+    HIP_TRY(hipSetDevice(h->device));
+    char *base = nullptr;
+    int rc = probe_scratch(h->device, &base);
+    if (rc) return rc;
+    std::vector<smp::ProbeComp> pc;
+    probe_table(h, lo, point, pc);
+    return probe_run(h, pc, (smp::ProbeComp *)base,
+                     (double *)(base + SMP_PROBE_NOALLOC_MAX_COMPS * sizeof(smp::ProbeComp)), out);
 }
 
 int smp_last_step_timing(const smp_problem *h, double *total_ms, double *substep_ms,

@@ -192,3 +192,20 @@ def test_adjoint_entry_points_need_a_device():
             s.reset()
     finally:
         s.close()
+
+
+def test_probe_noalloc_rejects_more_components_than_its_array():
+    """smp_probe_noalloc's fixed array holds SMP_PROBE_NOALLOC_MAX_COMPS = 4096
+    components: a 2D P90 problem (4186 components) is refused with
+    SMP_E_INVALID before any device call."""
+    from starmap_amd import problems as P
+    from starmap_amd.solver import HipSolver
+    s = HipSolver(I.starmap_init(P.gauss_periodic_2d(n=4, n_mom=90)), device=-1)
+    try:
+        assert s.n_sys == 4186
+        lo, pt, out = np.zeros(3), np.zeros(3), np.zeros(s.n_sys)
+        rc = _lib.lib().smp_probe_noalloc(s.handle, lo.ctypes.data, pt.ctypes.data, out.ctypes.data)
+        assert rc == _lib.SMP_E_INVALID
+        assert b"at most 4096 components" in _lib.lib().smp_last_error()
+    finally:
+        s.close()
