@@ -13,15 +13,20 @@ Prints ONE JSON line: value = FP64 moment-cell updates/s
 (algorithmic bytes per launch -- B_alg bytes per moment-cell update, the
 compulsory traffic of the reference's three-substep splitting, times the
 updates per launch -- over the mean launch time from HIP events on the
-library's stream), the FP32-state variant with its error against the FP64
-state after the same steps (each on the schedule the autotuner picks), and a CPU baseline (the FP64 oracle on the host
-cores, a bounded sample of the same workload).  A second leg ("gauss3d")
-measures the 3D P_7 (64 moments) 200^3 periodic Gaussian problem in FP64 and
-FP32 state; a third ("batched") 64 seeded 2D problems of uneven size, order
-and filter strength in FP32 state in the persistent batched kernel, against
-the FP64 plain rate and each problem's FP64 solve; a fourth ("adjoint") the
-cost of the discrete adjoint of 2000 steps of a 2D P_15 512^2 problem and
-its gradients against FP64 finite differences.
+library's stream; beside it the per-step fraction, the counter DRAM rate of
+the committed profile and the bandwidth of a plain device copy on this box),
+the FP32-state variant with its error against the FP64 state after the timed
+steps and after the workload's full 915 steps (each on the schedule the
+autotuner picks), and a CPU baseline (the FP64 oracle on the host's physical
+cores and on one thread, bounded samples of C2, C3 and the C5 forward
+problem, with the host's CPU model and core counts).  A second leg
+("gauss3d") measures the 3D P_7 (64 moments) 200^3 periodic Gaussian problem
+in FP64 and FP32 state; a third ("batched") 64 seeded 2D problems of uneven
+size, order and filter strength in FP32 state in the persistent batched
+kernel, against the FP64 plain rate and each problem's FP64 solve; a fourth
+("adjoint") the cost of the discrete adjoint of 2000 steps of a 2D P_15
+512^2 problem and its gradients (FP32 and FP64 state) against FP64 central
+finite differences at seeded cells.  --legs selects legs for record runs.
 """
 import argparse
 import json
@@ -86,9 +91,14 @@ def run_gpu(prob, precision, steps, warmup, tune=None):
 
 
 def fp32_error(s64, s32):
-    """Relative L2 per moment field of the FP32 state against the FP64 state
-    after the same steps (fields below 1e-8 of the largest are measured
-    against the largest), the worst field, and the error over all moments."""
+    """The FP32 state against the FP64 state after the same steps, in the
+    measures of the FP32 error bound: relative L2 over all moments; per
+    moment field relative to its own norm (the worst field, its norm over the
+    largest field's, and the worst among fields of at least 1e-2 of the
+    largest); every field's L2 error over the largest field's norm.  FP32
+    rounding noise is spatially white and reaches every moment order at a
+    similar absolute level, so fields far below the largest carry a larger
+    relative error."""
     norms, diffs = [], []
     for j in range(s64.n_sys):
         a = s64.get_state(j)
@@ -97,12 +107,16 @@ def fp32_error(s64, s32):
         diffs.append(np.linalg.norm(a - b))
     norms, diffs = np.array(norms), np.array(diffs)
     big = norms.max()
-    rel = np.where(norms > 1e-8 * big, diffs / np.maximum(norms, 1e-300), diffs / big)
+    rel = np.where(norms > 0, diffs / np.maximum(norms, 1e-300), np.where(diffs > 0, np.inf, 0.0))
+    large = norms >= 1e-2 * big
+    worst = int(rel.argmax())
     return {
-        "rel_l2_vs_fp64_max_over_moments": float(rel.max()),
         "rel_l2_vs_fp64_all_moments": float(np.sqrt((diffs ** 2).sum() / (norms ** 2).sum())),
-        "worst_moment": int(rel.argmax()) + 1,
-        "worst_moment_norm_over_max": float(norms[rel.argmax()] / big),
+        "rel_l2_vs_fp64_max_over_moments": float(rel.max()),
+        "worst_moment": worst + 1,
+        "worst_moment_norm_over_max": float(norms[worst] / big),
+        "rel_l2_vs_fp64_max_over_moments_above_1e-2_of_max": float(rel[large].max()),
+        "l2_err_over_largest_field_max": float(diffs.max() / big),
     }
 
 
@@ -145,6 +159,14 @@ def gauss3d_leg(n, n_mom, steps, warmup):
         log("3d %s: setting up %d^3 P%d" % (prec, n, n_mom))
         s, dt, wall, _ = run_gpu(prob, prec, steps, warmup)
         leg[prec] = kernel_roofline(s, steps, cells, wall, elem)
+        # counter DRAM bytes of the committed profile of this workload
+        tfile = os.path.join(ROOT, "profiles", "substep_kernel_p%d_%dx%dx%d_%s_traffic.json"
+                             % (n_mom, n, n, n, prec))
+        if os.path.exists(tfile):
+            with open(tfile) as f:
+                tr = json.load(f).get("hbm_bytes_per_launch")
+            leg[prec]["traffic"] = tr
+            leg[prec]["traffic_GBs"] = tr / (leg[prec]["mean_launch_ms"] / 1e3) / 1e9
         leg[prec]["schedule"] = getattr(s, "schedule", None)
         log("3d %s: %d steps in %.3f s" % (prec, steps, wall))
         ss[prec] = s
@@ -202,23 +224,27 @@ def batched_leg(fp64_plain_value, n_instances=64, check=True):
                          "candidates_ms": {"%s/%d/%d" % c: v for c, v in times.items()}},
         }
         if check:
-            worst, worst_all, worst_i = 0.0, 0.0, -1
+            # every FP32 problem against its FP64 solve: each error measure
+            # as the maximum over the problems, with the problem it comes from
+            errs = []
             for i, (p, s32) in enumerate(zip(pars, b.solvers)):
                 s64 = HipSolver(p, precision="fp64")
                 try:
                     dt = s64.prepare()
                     s64.step(dt, int(ns[i]))
-                    e = fp32_error(s64, s32)
+                    errs.append(fp32_error(s64, s32))
                 finally:
                     s64.close()
-                if e["rel_l2_vs_fp64_max_over_moments"] > worst:
-                    worst = e["rel_l2_vs_fp64_max_over_moments"]
-                    worst_i = i
-                worst_all = max(worst_all, e["rel_l2_vs_fp64_all_moments"])
-            leg["rel_l2_vs_fp64_max_over_problems_and_moments"] = worst
-            leg["rel_l2_vs_fp64_all_moments_max_over_problems"] = worst_all
-            leg["worst_problem"] = {"index": worst_i, "side": int(pars[worst_i]["n"][1]),
-                                    "n_mom": int(pars[worst_i]["n_mom"])}
+            fp32 = {}
+            for key in ("rel_l2_vs_fp64_all_moments", "rel_l2_vs_fp64_max_over_moments",
+                        "rel_l2_vs_fp64_max_over_moments_above_1e-2_of_max",
+                        "l2_err_over_largest_field_max"):
+                i = int(np.argmax([e[key] for e in errs]))
+                fp32[key] = {"max_over_problems": errs[i][key], "problem": i,
+                             "side": int(pars[i]["n"][1]), "n_mom": int(pars[i]["n_mom"])}
+            leg["fp32_vs_fp64"] = fp32
+            leg["problems_all_moments_above_1e-5"] = int(sum(
+                e["rel_l2_vs_fp64_all_moments"] > 1e-5 for e in errs))
     finally:
         b.close()
     return leg
@@ -283,56 +309,134 @@ def adjoint_leg(n=512, n_mom=15, steps=2000, every=50, fd_cells=8, seed=2027):
         "schedule": schedule,
     }
     if fd_cells > 0:
-        rng = np.random.default_rng(seed)
-        cells = rng.choice(n * n, size=fd_cells, replace=False)
-        sa0 = np.asarray(0.5 * np.exp(P.gaussian_random_field(n)))
-        ss0 = np.ones((n, n))
-        fd = {"a": [], "s": []}
+        leg["fd_check"] = adjoint_fd_check(prob, par, mask, n, n_mom, steps, every, fd_cells,
+                                           seed, (ga, gs))
+    return leg
+
+
+def adjoint_fd_check(prob, par, mask, n, n_mom, steps, every, fd_cells, seed, grads32,
+                     rel_step=1e-4):
+    """Per-cell dJ/dsigma_a and dJ/dsigma_s0 at seeded cells: the FP32-state
+    adjoint and an FP64-state adjoint against FP64 central differences of
+    the forward (relative step rel_step: its truncation error is about
+    rel_step^2 relative, and the rounding of J over the 2*rel_step*sigma
+    difference stays far below the gradients' size).  One FP64 handle is
+    re-used for every difference: it re-uploads the perturbed materials and
+    restarts from the initial state."""
+    from starmap_amd import init
+    from starmap_amd import problems as P
+    from starmap_amd.adjoint import Adjoint, objective_value
+    from starmap_amd.solver import HipSolver
+    rng = np.random.default_rng(seed)
+    cells = rng.choice(n * n, size=fd_cells, replace=False)
+    owner = dict(prob, sigma_a=(lambda x, y: prob["_owner"](x, y).astype(np.float64)))
+    s64 = Adjoint(par, precision="fp64")
+    try:
+        s64.run({0: mask}, n_steps=steps, every=every)
+        ga64, gs64 = s64.cell_gradients(owner, n * n)
+    finally:
+        s64.close()
+    sa0 = np.asarray(0.5 * np.exp(P.gaussian_random_field(n)))
+    ss0 = np.ones((n, n))
+    h = HipSolver(par, precision="fp64")
+    fd = {"a": [], "s": []}
+    try:
         for which in ("a", "s"):
             for c in cells:
                 vals = []
                 base = sa0 if which == "a" else ss0
-                eps = 1e-6 * base.flat[c]
+                eps = rel_step * base.flat[c]
                 for sgn in (1, -1):
                     a, b = sa0.copy(), ss0.copy()
                     (a if which == "a" else b).flat[c] += sgn * eps
-                    h64 = Adjoint(init.starmap_init(P.adjoint_grf(n=n, n_mom=n_mom, steps=steps,
-                                                                  sigma_a=a, sigma_s0=b)),
-                                  precision="fp64")
-                    try:
-                        d = h64.prepare()
-                        h64.step(d, steps)
-                        vals.append(objective_value(h64, mask))
-                    finally:
-                        h64.close()
+                    lifted = init.starmap_init(P.adjoint_grf(n=n, n_mom=n_mom, steps=steps,
+                                                             sigma_a=a, sigma_s0=b))
+                    h.par = dict(h.par, sigma_a=lifted["sigma_a"], sigma_s0=lifted["sigma_s0"])
+                    d = h.prepare()
+                    h.step(d, steps)
+                    vals.append(objective_value(h, mask))
                 fd[which].append((vals[0] - vals[1]) / (2 * eps))
             log("adjoint: finite differences of sigma_%s done" % which)
-        res = {}
-        for which, g in (("a", ga), ("s", gs)):
-            f = np.array(fd[which])
-            ad = g[cells]
-            scale = np.abs(f).max()
-            res["sigma_" + which] = {
-                "max_abs_err_over_max_fd": float(np.abs(ad - f).max() / scale) if scale else None,
-                "adjoint": ad.tolist(), "finite_difference": f.tolist()}
-        leg["fd_check"] = dict(res, cells=[int(c) for c in cells], rel_step=1e-6,
-                               forward_precision="fp64")
-    return leg
+    finally:
+        h.close()
+    res = {"cells": [int(c) for c in cells], "rel_step": rel_step, "forward_precision": "fp64"}
+    for which, g32, g64 in (("a", grads32[0], ga64), ("s", grads32[1], gs64)):
+        f = np.array(fd[which])
+        a32, a64 = g32[cells], g64[cells]
+        res["sigma_" + which] = {
+            "finite_difference": f.tolist(),
+            "adjoint_fp64_state": a64.tolist(),
+            "adjoint_fp32_state": a32.tolist(),
+            "max_rel_err_fp64_adjoint_vs_fd": float(np.max(np.abs(a64 - f) / np.abs(f))),
+            "max_rel_err_fp32_adjoint_vs_fd": float(np.max(np.abs(a32 - f) / np.abs(f))),
+            "max_rel_diff_fp32_vs_fp64_adjoint": float(np.max(np.abs(a32 - a64) / np.abs(a64))),
+            "max_abs_err_fp32_adjoint_over_max_fd": float(np.abs(a32 - f).max() / np.abs(f).max()),
+        }
+    return res
 
 
-def cpu_baseline(prob, steps):
+def cpu_inventory():
+    """nproc, the CPUs this process may run on, their model, sockets and
+    physical cores (from /proc/cpuinfo)."""
+    usable = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else []
+    model, cores, sockets = None, set(), set()
+    try:
+        cur = {}
+        for line in open("/proc/cpuinfo"):
+            if not line.strip():
+                if cur.get("processor") is not None and (not usable or int(cur["processor"]) in usable):
+                    cores.add((cur.get("physical id", "0"), cur.get("core id", cur["processor"])))
+                    sockets.add(cur.get("physical id", "0"))
+                    model = model or cur.get("model name")
+                cur = {}
+                continue
+            k, _, v = line.partition(":")
+            cur[k.strip()] = v.strip()
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "usable_cpus": len(usable) or os.cpu_count(),
+            "cpu_model": model, "sockets": len(sockets) or None,
+            "physical_cores": len(cores) or None}
+
+
+def cpu_leg(workload, steps, threads):
+    """oracle/cpu_baseline.py in a subprocess with OMP_NUM_THREADS=threads."""
+    import subprocess
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    r = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline", workload, str(steps)],
+                       cwd=ROOT, env=env, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError("cpu baseline %s failed: %s" % (workload, r.stderr[-2000:]))
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline(c2_steps=10, c2_steps_1t=1):
     """The FP64 oracle (C restatement of the reference step, OpenMP over
-    components) timed on the host: `steps` full steps of the same workload."""
-    from oracle import starmap_oracle as so
-    par = so.starmap_init(prob)
-    # a final time just past `steps` steps keeps dt = dt_hyp*cn
-    lam = np.max(np.abs(np.linalg.eigvals(par["Mz"])))
-    h = (par["ax"][3] - par["ax"][2]) / par["n"][1]
-    dt = h / lam / 2 * par["cn"]
-    par = dict(par, tfinal=(steps - 0.5) * dt, t_plot=[])
-    _, p2 = so.starmap_solver(par, verbose=False, return_state=True)
-    cpu = p2["_oracle"]["cputime"]
-    return p2["_oracle"]["step_count"], float(cpu[0])
+    components) on the host cores: K full steps of C2 with one thread per
+    physical core and with one thread, extrapolated to C2's 915 steps;
+    three and one steps of C3 (3D P7 200^3), 10 and 2 steps of the C5
+    forward problem (2D P15 512^2).  The stepping loop only."""
+    inv = cpu_inventory()
+    nt = int(inv["physical_cores"] or inv["usable_cpus"] or 1)
+    legs = {}
+    for name, wl, k_all, k_one in (("c2", "c2", c2_steps, c2_steps_1t), ("c3", "c3", 3, 1),
+                                   ("c5", "c5", 10, 2)):
+        log("cpu baseline: %s, %d steps on %d threads and %d on 1" % (name, k_all, nt, k_one))
+        legs[name] = {"all_physical_cores": cpu_leg(wl, k_all, nt),
+                      "one_thread": cpu_leg(wl, k_one, 1)}
+    return inv, nt, legs
+
+
+def copy_ceiling():
+    """Bandwidth of a plain 4 GiB device copy on this GPU (read plus write;
+    starmap_amd/profiling.py copybw, run in its own process)."""
+    import subprocess
+    r = subprocess.run([sys.executable, "-m", "starmap_amd.profiling", "copybw"], cwd=ROOT,
+                       capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        log("copy ceiling failed: " + r.stderr[-500:])
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])["copy_GBs"]
 
 
 def main():
@@ -343,6 +447,9 @@ def main():
     ap.add_argument("--n", type=int, default=1000, help="grid side")
     ap.add_argument("--n-mom", type=int, default=39)
     ap.add_argument("--precision", choices=["fp64", "fp32", "both"], default="both")
+    ap.add_argument("--legs", default="c2,c3,batch,adjoint,cpu",
+                    help="comma-separated legs: c2 (the metric line; needed by the others' "
+                         "ratios), c3, batch, adjoint, cpu")
     # about 12 s of the oracle's stepping on 16 host threads
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -366,10 +473,59 @@ def main():
         print("bench.py runs on one GPU only: the time-stepping core is a single-GPU "
               "kernel path (--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    legs = set(args.legs.split(","))
+    if args.no_cpu_baseline:
+        legs.discard("cpu")
+    if args.c3_steps <= 0:
+        legs.discard("c3")
+    if args.batch <= 0:
+        legs.discard("batch")
+    if args.adjoint_steps <= 0:
+        legs.discard("adjoint")
 
     from starmap_amd import problems as P
+    from starmap_amd.batch import count_steps
     prob = P.lattice_2d(n=args.n, n_mom=args.n_mom, tfinal=3.2)
     cells = args.n * args.n
+    out = {"metric": METRIC}
+    if "c2" in legs:
+        out = c2_leg(args, prob, cells, count_steps)
+    if "c3" in legs:
+        out["gauss3d"] = gauss3d_leg(args.c3_n, 7, args.c3_steps, min(args.warmup, 5))
+    if "batch" in legs:
+        out["batched"] = batched_leg(out.get("value") if out.get("dtype") == "f64" else None,
+                                     n_instances=args.batch, check=not args.no_batch_check)
+    if "adjoint" in legs:
+        out["adjoint"] = adjoint_leg(steps=args.adjoint_steps, fd_cells=args.fd_cells)
+    if "cpu" in legs:
+        inv, nt, cl = cpu_baseline(c2_steps=args.cpu_steps)
+        c2 = cl["c2"]["all_physical_cores"]
+        out["cpu_baseline"] = {
+            "value": c2["mcu_per_s"],
+            "unit": "moment-cell updates/s",
+            "cores": nt,
+            "kind": "port",
+            "sample": "FP64 oracle (C restatement of starmap_solver.m:307-421, OpenMP over "
+                      "components; set-up excluded) on %d threads, one per physical core: %d "
+                      "full steps of the 2D P39 lattice 1000x1000 in %.1f s, extrapolated to "
+                      "its 915 steps: %.0f s" % (nt, c2["steps"], c2["seconds"],
+                                                   c2["full_seconds_extrapolated"]),
+            "host": inv,
+            "one_thread": {"value": cl["c2"]["one_thread"]["mcu_per_s"],
+                           "steps": cl["c2"]["one_thread"]["steps"],
+                           "seconds": cl["c2"]["one_thread"]["seconds"],
+                           "full_seconds_extrapolated":
+                               cl["c2"]["one_thread"]["full_seconds_extrapolated"]},
+            "legs": cl,
+        }
+        if out.get("value"):
+            out["cpu_baseline"]["gpu_over_cpu"] = out["value"] / c2["mcu_per_s"]
+    print(json.dumps(out), flush=True)
+
+
+def c2_leg(args, prob, cells, count_steps):
+    """The metric line: C2 (2D P39 lattice 1000^2) in FP64 and FP32, the
+    substep kernel's roofline, and the FP32 error after the full run."""
     res = {}
     if args.precision in ("fp64", "both"):
         log("fp64: setting up %dx%d P%d" % (args.n, args.n, args.n_mom))
@@ -408,6 +564,9 @@ def main():
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+    log("copy ceiling: 4 GiB device copy")
+    copy_gbs = copy_ceiling()
+    step_s = wall / args.steps
     out = {
         "metric": METRIC,
         "value": mcu / wall,
@@ -415,7 +574,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": step_s * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -445,8 +604,18 @@ def main():
             "kernel_bytes_per_launch": kernel_bytes_per_launch,
             "kernel_bytes_GBs": kernel_bytes_per_launch / mean_launch_s / 1e9,
             "kernel_bytes_frac": kernel_bytes_per_launch / mean_launch_s / 1e9 / HBM_PEAK_GBS,
-            # the whole-step rate (wall clock, host gaps included) in B_alg terms
+            # the whole-step rate (wall clock, host gaps included) in B_alg
+            # terms: per step, not per launch
             "b_alg_wall_GBs": bpm * mcu / wall / 1e9,
+            "frac_per_step": bpm * n_sys * cells / step_s / 1e9 / HBM_PEAK_GBS,
+            # the DRAM bytes the counters measured for this launch (the
+            # committed profile) over this run's mean launch time, and the
+            # bandwidth a plain device copy reaches on this box
+            "traffic_GBs": traffic / mean_launch_s / 1e9 if traffic else None,
+            "traffic_frac": traffic / mean_launch_s / 1e9 / HBM_PEAK_GBS if traffic else None,
+            "copy_ceiling_GBs": copy_gbs,
+            "traffic_over_copy_ceiling": (traffic / mean_launch_s / 1e9 / copy_gbs
+                                          if traffic and copy_gbs else None),
         },
     }
     if "fp32" in res and "fp64" in res:
@@ -460,33 +629,22 @@ def main():
             "speedup_vs_fp64": (mcu / wall32) / out["value"],
             "roofline_achieved_GBs": ach32,
             "roofline_frac": ach32 / HBM_PEAK_GBS,
+            "mean_launch_ms": sub32 / l32,
             "schedule": getattr(s32, "schedule", None),
+            "error_after_steps": args.warmup + args.steps,
         }, **err)
+        # both states on to the workload's full length (915 steps to
+        # t_final 3.2), then the FP32 error again
+        full = count_steps(dt, prob["tfinal"])
+        done = args.warmup + args.steps
+        if full > done:
+            log("fp32 accuracy: both states on to step %d" % full)
+            s64.step(dt, full - done)
+            s32.step(dt, full - done)
+            out["fp32"]["full_run"] = dict(fp32_error(s64, s32), steps=full)
     for v in res.values():
         v[0].close()
-    if args.c3_steps > 0:
-        out["gauss3d"] = gauss3d_leg(args.c3_n, 7, args.c3_steps, min(args.warmup, 5))
-    if args.batch > 0:
-        out["batched"] = batched_leg(out["value"] if main_key == "fp64" else None,
-                                     n_instances=args.batch, check=not args.no_batch_check)
-    if args.adjoint_steps > 0:
-        out["adjoint"] = adjoint_leg(steps=args.adjoint_steps, fd_cells=args.fd_cells)
-
-    if not args.no_cpu_baseline:
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        log("cpu baseline: %d steps of the oracle on %d threads" % (args.cpu_steps, threads))
-        nsteps, secs = cpu_baseline(prob, args.cpu_steps)
-        out["cpu_baseline"] = {
-            "value": n_sys * cells * nsteps / secs,
-            "unit": "moment-cell updates/s",
-            "cores": threads,
-            "kind": "port",
-            "sample": "FP64 oracle (C restatement of starmap_solver.m:307-421, OpenMP over "
-                      "components), %d full steps of the same %dx%d P%d workload; stepping "
-                      "loop only, %.1f s" % (nsteps, args.n, args.n, args.n_mom, secs),
-        }
-        out["cpu_baseline"]["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
-    print(json.dumps(out), flush=True)
+    return out
 
 
 if __name__ == "__main__":

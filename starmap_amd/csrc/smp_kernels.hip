@@ -282,17 +282,23 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     // byte address of this thread's cell in the first source slot; term
     // offsets are byte offsets from the start of LDS
     const int lb = lbase * (int)sizeof(T);
-    // The update computes in the state precision (the FP32 state's flux
-    // sum and half-steps in FP32; the term coefficients of its records are
-    // FP32 too).
+    // The flux sum W computes in the state precision (FP32 terms with FP32
+    // coefficients for the FP32 state); the half-step update and the filter
+    // factor in FP64 for both states.  Scalar coefficients -- the decay
+    // rate, its factor E, dt/2 and the filter factors -- are then used
+    // unrounded: rounded to FP32 once and applied every half-step, each would
+    // bias the FP32 state by the same relative amount per application, an
+    // error that grows linearly with the step count (about 1e-5 after 500
+    // filtered steps).  With packed FP32 instructions off, the FP64 update
+    // costs no more issue slots than the FP32 one.
     typedef T A;
-    const A hdt = (A)half_dt;
+    typedef double D;
     // coefficient: staged field (off >= 0), scalar (off == -1) or, in the
     // GEN variant only, a field read from global memory (off == -2)
-    auto coef = [&](int off, const void *p, double s) -> A {
-        const A staged = (A)cfa[(off > 0 ? off : 0) + ctid];
-        A v = off >= 0 ? staged : (A)s;
-        if (GEN && off == -2) v = in_store ? (A)glb(reinterpret_cast<const T *>(p))[gidx] : (A)0;
+    auto coef = [&](int off, const void *p, double s) -> D {
+        const D staged = (D)cfa[(off > 0 ? off : 0) + ctid];
+        D v = off >= 0 ? staged : s;
+        if (GEN && off == -2) v = in_store ? (D)glb(reinterpret_cast<const T *>(p))[gidx] : 0.0;
         return v;
     };
     // Every group holds GMAX target records; unused ones are zero records
@@ -315,23 +321,23 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
                 w += c * ((A)*reinterpret_cast<const T *>(lo + tstride) -
                           (A)*reinterpret_cast<const T *>(lo));
             }
-            const A sig = coef(r.sig_off, r.sig_p, r.sig_s);
-            const A E = coef(r.E_off, r.E_p, r.E_s);
-            const A wq = w + coef(r.Q_off, r.Q_p, 0.0);
-            A u = (A)u_in[i];
+            const D sig = coef(r.sig_off, r.sig_p, r.sig_s);
+            const D E = coef(r.E_off, r.E_p, r.E_s);
+            const D wq = (D)w + coef(r.Q_off, r.Q_p, 0.0);
+            D u = (D)u_in[i];
 #pragma unroll
             for (int app = 0; app < NAPP; ++app) {
                 // U + dt/2*(W+Q - s.*U).*E   (starmap_solver.m:352-354)
-                u = u + (hdt * (wq - sig * u)) * E;
+                u = u + (half_dt * (wq - sig * u)) * E;
                 const bool s1 = (app == 0 ? fslot0 : fslot1) != 0;
-                A f = (A)(s1 ? r.F_s[1] : r.F_s[0]);   // 1 for targets without a filter
+                D f = s1 ? r.F_s[1] : r.F_s[0];   // 1 for targets without a filter
                 if (GEN && r.filt) {
                     const void *fp = s1 ? r.F[1] : r.F[0];
-                    if (fp && act) f = (A)glb(reinterpret_cast<const T *>(fp))[gidx];
+                    if (fp && act) f = (D)glb(reinterpret_cast<const T *>(fp))[gidx];
                 }
                 // the state after every half-step is a value of the state
                 // precision, as if stored: fused and separate launches agree
-                u = (A)(T)(u * f);
+                u = (D)(T)(u * f);
             }
             SMP_GLOBAL T *dst = act ? reinterpret_cast<SMP_GLOBAL T *>(ug_st + r.uoff) : sink + tid;
             *dst = (T)u;

@@ -5,6 +5,15 @@
     ``rocprofv3 --pmc FETCH_SIZE`` / ``--pmc WRITE_SIZE`` it gives the factor
     between each counter and the bytes a kernel of known size moves.
 
+``python -m starmap_amd.profiling copybw``
+    prints the bandwidth of a 4 GiB device copy (best of five, events).
+
+``python -m starmap_amd.profiling leg WORKLOAD PRECISION STEPS [TX TY TZ BPC CW]``
+    steps one benchmark workload on cuda:0 for the profiler, on a stated
+    schedule (default: the library's): WORKLOAD is ``c2`` (2D P39 lattice
+    1000^2) or ``c3`` (3D P7 Gaussian 200^3 periodic), PRECISION ``fp64`` or
+    ``fp32``.  Two warm-up steps, then STEPS steps in one ``smp_step`` call.
+
 ``python -m starmap_amd.profiling summarize KERNEL_DIR FETCH_DIR WRITE_DIR
 COPYF_DIR COPYW_DIR [KERNEL]``
     reads the rocprofv3 CSV outputs (kernel-trace stats, the FETCH_SIZE and
@@ -30,6 +39,51 @@ def copycal():
         b.copy_(a)
     torch.cuda.synchronize()
     print("copied 1 GiB three times", flush=True)
+
+
+def copybw(gib=4, reps=5):
+    """Bandwidth of a plain device-to-device copy of `gib` GiB (read plus
+    write), the best of `reps` runs timed with events on the copy's stream:
+    the HBM rate a streaming kernel reaches on this GPU."""
+    import torch
+    n = gib * 2 ** 30 // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    best = float("inf")
+    for _ in range(reps + 1):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    print(json.dumps({"copy_GBs": 2 * n * 8 / (best / 1e3) / 1e9, "bytes": 2 * n * 8,
+                      "best_ms": best}), flush=True)
+
+
+def leg(workload, precision, steps, tile=None, bpc=0, cw=0):
+    import time
+    from . import init, problems
+    from .solver import HipSolver
+    if workload == "c2":
+        prob = problems.lattice_2d(n=1000, n_mom=39, tfinal=3.2)
+    elif workload == "c3":
+        prob = problems.gauss_periodic_3d(n=200, n_mom=7, tfinal=0.75)
+    else:
+        raise SystemExit("workload must be c2 or c3")
+    s = HipSolver(init.starmap_init(prob), precision=precision)
+    dt = s.prepare()
+    if tile is not None:
+        s.set_schedule(tile, bpc, cw)
+    s.step(dt, 2)
+    t0 = time.perf_counter()
+    s.step(dt, steps)
+    wall = time.perf_counter() - t0
+    tot, sub, nl, by = s.timing()
+    print(json.dumps({"workload": workload, "precision": precision, "steps": steps,
+                      "schedule": s.get_schedule(), "wall_s": wall, "substep_ms": sub,
+                      "launches": nl, "mean_launch_ms": sub / nl, "staged_bytes": by}), flush=True)
+    s.close()
 
 
 def _rows(d, pattern):
@@ -84,6 +138,13 @@ def summarize(kdir, fdir, wdir, cfdir, cwdir, kernel="substep_kernel"):
 if __name__ == "__main__":
     if len(sys.argv) >= 2 and sys.argv[1] == "copycal":
         copycal()
+    elif len(sys.argv) >= 2 and sys.argv[1] == "copybw":
+        copybw()
+    elif len(sys.argv) in (5, 10) and sys.argv[1] == "leg":
+        extra = [int(v) for v in sys.argv[5:]]
+        leg(sys.argv[2], sys.argv[3], int(sys.argv[4]),
+            tile=extra[:3] if extra else None, bpc=extra[3] if extra else 0,
+            cw=extra[4] if extra else 0)
     elif len(sys.argv) in (7, 8) and sys.argv[1] == "summarize":
         summarize(*sys.argv[2:])
     else:
