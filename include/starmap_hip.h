@@ -106,6 +106,9 @@ int smp_create(const smp_desc *desc, smp_problem **out);
 /* Release all device memory of the handle. */
 int smp_destroy(smp_problem *h);
 
+/* Number of moment components of the handle, length(par.Mz). */
+int smp_num_components(const smp_problem *h, int32_t *n_sys);
+
 /* Grid type (1 = cell centres, 2 = faces) of each component per solver axis,
  * gt[3*j+d] -- gtx/gty/gtz of starmap_solver.m:95-98. */
 int smp_grid_types(const smp_problem *h, int32_t *gt);

@@ -28,7 +28,7 @@ EXPORTS = ["smp_last_error", "smp_version", "smp_create", "smp_destroy", "smp_gr
            "smp_checkpoint_free", "smp_batch_create", "smp_batch_destroy", "smp_batch_run",
            "smp_batch_timing", "smp_adjoint_reset", "smp_adjoint_seed", "smp_adjoint",
            "smp_adjoint_get", "smp_adjoint_timing", "smp_moment_norms", "smp_probe",
-           "smp_check_finite", "smp_probe_noalloc"]
+           "smp_check_finite", "smp_probe_noalloc", "smp_num_components"]
 
 
 class SmpError(RuntimeError):
@@ -106,6 +106,7 @@ def lib():
         "smp_probe": (ctypes.c_int, [vp, vp, vp, vp]),
         "smp_check_finite": (ctypes.c_int, [vp, vp]),
         "smp_probe_noalloc": (ctypes.c_int, [vp, vp, vp, vp]),
+        "smp_num_components": (ctypes.c_int, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

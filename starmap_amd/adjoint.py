@@ -10,8 +10,10 @@ sigma_sm(m) for j > 1, sigma_a for j = 1, starmap_solver.m:214-252), and
 points to owning cells that the material functions use.
 
 ``StarmapObjective`` is a ``torch.autograd.Function`` over the same library:
-per-cell sigma_a and sigma_s0 in, the objective out, gradients from the
-adjoint in backward.
+per-cell sigma_a and sigma_s0 in, the strip objective out, gradients from the
+adjoint.  ``StarmapFinalMoment`` returns the final zeroth moment instead, so
+that a torch loss of it drives the adjoint: its backward seeds dJ/dU1(T)
+from torch.
 """
 import ctypes
 
@@ -142,5 +144,50 @@ try:
             g = gout.detach().double().cpu()
             return ((ga * g).to(ctx.dev, ctx.dtype), (gs * g).to(ctx.dev, ctx.dtype),
                     None, None, None)
+
+    class StarmapFinalMoment(torch.autograd.Function):
+        """U1(T), the zeroth moment at the final time on the n x n cell
+        centres, as a function of per-cell sigma_a and sigma_s0 tensors, so
+        that any torch loss of it can be differentiated: forward runs the
+        time stepper only; backward seeds the adjoint with the incoming
+        dJ/dU1(T) and returns dJ/dsigma_a, dJ/dsigma_s0 per cell from
+        smp_adjoint (which recomputes the forward with checkpoints).
+        make_prob(sigma_a, sigma_s0) builds the problem from numpy arrays
+        and must give it an ``_owner`` map from points to cells."""
+
+        @staticmethod
+        def forward(ctx, sigma_a, sigma_s0, make_prob, precision="fp32", every=50):
+            sa = sigma_a.detach().cpu().double().numpy()
+            ss = sigma_s0.detach().cpu().double().numpy()
+            s = Adjoint(starmap_init(make_prob(sa, ss)), precision=precision)
+            try:
+                dt = s.prepare()
+                n_steps = count_steps(dt, s.par["tfinal"])
+                s.step(dt, n_steps)
+                U1 = s.get_state(0)
+            finally:
+                s.close()
+            ctx.args = (sa, ss, make_prob, precision, every, n_steps)
+            ctx.dev, ctx.dtype = sigma_a.device, sigma_a.dtype
+            return torch.from_numpy(np.ascontiguousarray(U1.reshape(sa.shape, order="F"))).to(
+                sigma_a.device)
+
+        @staticmethod
+        def backward(ctx, gU):
+            sa, ss, make_prob, precision, every, n_steps = ctx.args
+            prob = make_prob(sa, ss)
+            s = Adjoint(starmap_init(prob), precision=precision)
+            try:
+                seed = np.asarray(gU.detach().double().cpu().numpy())
+                s.run({0: seed.reshape(s.shape(0), order="F")}, n_steps=n_steps, every=every)
+                n = sa.shape[0]
+                owner_prob = dict(prob, sigma_a=(lambda x, y: prob["_owner"](x, y).astype(np.float64)))
+                ga, gs = s.cell_gradients(owner_prob, n * n)
+            finally:
+                s.close()
+            return (torch.from_numpy(ga.reshape(sa.shape)).to(ctx.dev, ctx.dtype),
+                    torch.from_numpy(gs.reshape(ss.shape)).to(ctx.dev, ctx.dtype),
+                    None, None, None)
 except ImportError:          # torch is optional for the gradient itself
     StarmapObjective = None
+    StarmapFinalMoment = None

@@ -117,6 +117,10 @@ struct smp_problem {
     // states recorded while a segment is recomputed
     void *lam = nullptr, *mu = nullptr, *G = nullptr;
     hipStream_t adj_stream = nullptr;   // the adjoint's recomputation (default priority)
+    // the adjoint's events, created once: timing (forward start, reverse
+    // start, end) and ordering between the two streams (forward done,
+    // recomputation and reverse sweep of each record set)
+    hipEvent_t adj_ev[8] = {};
     int adj_ngrp[2] = {0, 0};     // odd, even: grid types, list ranges, codes
     int32_t adj_grp0[2][9] = {};
     int32_t adj_gcode[2][8] = {};
@@ -133,6 +137,8 @@ struct smp_problem {
         if (adj_stream) (void)hipStreamSynchronize(adj_stream);
         for (void *p : allocs) (void)hipFree(p);
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        for (hipEvent_t e : adj_ev)
+            if (e) (void)hipEventDestroy(e);
         if (adj_stream) (void)hipStreamDestroy(adj_stream);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -556,6 +562,12 @@ int smp_destroy(smp_problem *h) {
         if (h->stream) (void)hipStreamSynchronize(h->stream);
     }
     delete h;
+    return SMP_OK;
+}
+
+int smp_num_components(const smp_problem *h, int32_t *n_sys) {
+    if (!h || !n_sys) return fail(SMP_E_INVALID, "null argument");
+    *n_sys = h->geo.n_sys;
     return SMP_OK;
 }
 
@@ -1239,10 +1251,12 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
         HIP_TRY(hipMemsetAsync(p, 0, st, h->stream));
         h->adj_rec.push_back(p);
     }
-    hipEvent_t e0, e1, e2;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    HIP_TRY(hipEventCreate(&e2));
+    for (int q = 0; q < 8; ++q)
+        if (!h->adj_ev[q])
+            HIP_TRY(q < 3 ? hipEventCreate(&h->adj_ev[q])
+                          : hipEventCreateWithFlags(&h->adj_ev[q], hipEventDisableTiming));
+    hipEvent_t e0 = h->adj_ev[0], e1 = h->adj_ev[1], e2 = h->adj_ev[2], ev_fwd = h->adj_ev[3];
+    hipEvent_t ev_rc[2] = {h->adj_ev[4], h->adj_ev[5]}, ev_rev[2] = {h->adj_ev[6], h->adj_ev[7]};
     HIP_TRY(hipEventRecord(e0, h->stream));
     // forward pass: segment checkpoints, the state ends at t = n_steps*dt
     for (int64_t sg = 0; sg < n_seg; ++sg) {
@@ -1302,12 +1316,6 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
     // on the handle's stream.
     if (!h->adj_stream) HIP_TRY(hipStreamCreateWithFlags(&h->adj_stream, hipStreamNonBlocking));
     hipStream_t rs = h->adj_stream;
-    hipEvent_t ev_fwd, ev_rc[2], ev_rev[2];
-    HIP_TRY(hipEventCreateWithFlags(&ev_fwd, hipEventDisableTiming));
-    for (int q = 0; q < 2; ++q) {
-        HIP_TRY(hipEventCreateWithFlags(&ev_rc[q], hipEventDisableTiming));
-        HIP_TRY(hipEventCreateWithFlags(&ev_rev[q], hipEventDisableTiming));
-    }
     HIP_TRY(hipEventRecord(ev_fwd, h->stream));
     HIP_TRY(hipStreamWaitEvent(rs, ev_fwd, 0));
     const smp::SubstepArgs<T> f_odd_first = make_args<T>(h, 0, dt, 1, 0, 0);
@@ -1380,14 +1388,6 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
     HIP_TRY(hipEventElapsedTime(&r, e1, e2));
     h->adj_fwd_ms = f;
     h->adj_rev_ms = r;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(e2);
-    (void)hipEventDestroy(ev_fwd);
-    for (int q = 0; q < 2; ++q) {
-        (void)hipEventDestroy(ev_rc[q]);
-        (void)hipEventDestroy(ev_rev[q]);
-    }
     return SMP_OK;
 }
 

@@ -209,3 +209,17 @@ def test_probe_noalloc_rejects_more_components_than_its_array():
         assert b"at most 4096 components" in _lib.lib().smp_last_error()
     finally:
         s.close()
+
+
+def test_matlab_gateway_matches_header_and_wrapper():
+    """The MEX gateway (not compiled here: no MATLAB in this image) calls only
+    entry points the header declares, and the M-side wrapper sends only
+    commands the gateway implements."""
+    mdir = os.path.join(ROOT, "starmap_amd", "matlab")
+    c_src = open(os.path.join(mdir, "starmap_hip_mex.c")).read()
+    m_src = open(os.path.join(mdir, "starmap_solver_hip.m")).read()
+    called = set(re.findall(r"\b(smp_[a-z_0-9]+)\s*\(", c_src))
+    assert called and called <= set(_declared()), called - set(_declared())
+    commands = set(re.findall(r'strcmp\(cmd, "([a-z_0-9]+)"\)', c_src))
+    sent = set(re.findall(r"starmap_hip_mex\('([a-z_0-9]+)'", m_src))
+    assert sent and sent <= commands, sent - commands

@@ -24,6 +24,8 @@ from oracle import starmap_oracle as so
 from starmap_amd import init as I
 from starmap_amd import problems as P
 from starmap_amd.adjoint import Adjoint, objective_value, strip_objective
+from starmap_amd.batch import count_steps
+from starmap_amd.solver import HipSolver
 
 pytestmark = pytest.mark.gpu
 
@@ -248,3 +250,56 @@ def test_dot_product_every_component():
     scale = sum(float(np.sum(np.abs(lam[j] * d[j]))) for j in d)
     print("dot product", dJ, ip, scale, flush=True)
     assert abs(dJ - ip) <= 1e-11 * scale
+
+
+def test_torch_loss_of_final_moment_drives_the_adjoint():
+    """StarmapFinalMoment: U1(T) as a torch tensor; a nonlinear torch loss of
+    it, L = sum(w * U1^2) + sum(U1), differentiated by autograd (backward
+    seeds the adjoint with dL/dU1(T) = 2 w U1 + 1) against Richardson-
+    extrapolated FP64 central differences of L through the forward at seeded
+    cells."""
+    torch = pytest.importorskip("torch")
+    from starmap_amd.adjoint import StarmapFinalMoment
+    n = 12
+    prob0, sa, ss = small(n=n)
+
+    def make_prob(a, b):
+        return P.adjoint_grf(n=n, n_mom=3, steps=9, sigma_a=a, sigma_s0=b, src_std=0.2)
+
+    w = np.random.default_rng(5).uniform(0.5, 1.5, size=(n, n))
+
+    def loss_np(a, b):
+        s = HipSolver(I.starmap_init(make_prob(a, b)), precision="fp64")
+        try:
+            dt = s.prepare()
+            s.step(dt, count_steps(dt, s.par["tfinal"]))
+            U1 = s.get_state(0).reshape(n, n, order="F")
+        finally:
+            s.close()
+        return float(np.sum(w * U1 ** 2) + np.sum(U1))
+
+    ta = torch.tensor(sa, requires_grad=True)
+    ts = torch.tensor(ss, requires_grad=True)
+    U1 = StarmapFinalMoment.apply(ta, ts, make_prob, "fp64", 4)
+    L = (torch.tensor(w) * U1 ** 2).sum() + U1.sum()
+    assert abs(float(L) - loss_np(sa, ss)) <= 1e-12 * abs(float(L))
+    L.backward()
+    for c in np.random.default_rng(9).choice(n * n, size=4, replace=False):
+        for grad, which in ((ta.grad, "a"), (ts.grad, "s")):
+            base = sa if which == "a" else ss
+
+            def central(eps):
+                vals = []
+                for sgn in (1, -1):
+                    a, b = sa.copy(), ss.copy()
+                    (a if which == "a" else b).flat[c] += sgn * eps
+                    vals.append(loss_np(a, b))
+                return (vals[0] - vals[1]) / (2 * eps)
+            # Richardson extrapolation of two central differences: the
+            # O(eps^2) truncation error of the quadratic loss cancels; what
+            # is left is the rounding of L over the 2*eps difference, about
+            # 3e-11 absolute here (the adjoint is exact: cells with
+            # gradients of 1e-3 agree to 1e-9)
+            e = 1e-4 * base.flat[c]
+            fd = (4 * central(e) - central(2 * e)) / 3
+            assert abs(float(grad.numpy().flat[c]) - fd) <= 1e-7 * abs(fd) + 1e-10, (which, c)
