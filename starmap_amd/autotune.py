@@ -31,6 +31,13 @@ FIT_TILES = {1: [(255, 1, 1)],
              2: [(63, 4, 1), (127, 2, 1), (31, 8, 1), (15, 16, 1)],
              3: [(15, 4, 4), (31, 4, 2), (15, 8, 2)]}
 ROW_COST = {16: 1.33, 32: 1.15, 64: 1.0, 128: 0.92, 256: 0.9}
+# tiles of up to 512 cells, two neighbouring cells per thread: FP32 state
+# only; "fit-pair" picks one of them per handle of a batch
+PAIR_TILES = {2: [(63, 8, 1), (127, 4, 1), (255, 2, 1)],
+              3: [(31, 4, 4), (15, 8, 4)]}
+FIT_PAIR = "fit-pair"
+FIT_PAIR_TILES = {2: [(63, 8, 1), (127, 4, 1), (255, 2, 1), (31, 16, 1), (15, 32, 1)],
+                  3: [(31, 4, 4), (15, 8, 4)]}
 BLOCKS = (1, 2)
 CELL_WAVES = (4, 8)
 _SLOT = 4095            # checkpoint slot used while tuning
@@ -84,7 +91,11 @@ def candidates(target):
     tiles = list(TILES[dim])
     if hasattr(target, "solvers"):
         tiles.append(FIT)
-    return [(t, b, c) for t in tiles for b in BLOCKS for c in CELL_WAVES]
+    out = [(t, b, c) for t in tiles for b in BLOCKS for c in CELL_WAVES]
+    if dim > 1 and all(s.precision == "fp32" for s in _solvers(target)):
+        pt = list(PAIR_TILES[dim]) + ([FIT_PAIR] if hasattr(target, "solvers") else [])
+        out += [(t, b, c) for t in pt for b in BLOCKS for c in CELL_WAVES]
+    return out
 
 
 def staged_elements(s, tile):
@@ -99,18 +110,20 @@ def staged_elements(s, tile):
     return tiles * slot
 
 
-def fit_tiles(s):
-    """FIT_TILES of the handle's dimension, lowest estimated cost first."""
-    return sorted(FIT_TILES[_dim(s)], key=lambda t: staged_elements(s, t) * ROW_COST[t[0] + 1])
+def fit_tiles(s, pair=False):
+    """FIT_TILES (or FIT_PAIR_TILES) of the handle's dimension, lowest
+    estimated cost first."""
+    table = FIT_PAIR_TILES if pair else FIT_TILES
+    return sorted(table[_dim(s)], key=lambda t: staged_elements(s, t) * ROW_COST[t[0] + 1])
 
 
 def apply(target, tile, blocks_per_cu, cell_waves):
     for s in _solvers(target):
-        if tile != FIT:
+        if tile not in (FIT, FIT_PAIR):
             s.set_schedule(tile, blocks_per_cu, cell_waves)
             continue
         err = None
-        for t in fit_tiles(s):
+        for t in fit_tiles(s, pair=tile == FIT_PAIR):
             try:
                 s.set_schedule(t, blocks_per_cu, cell_waves)
                 err = None
@@ -137,7 +150,7 @@ def tune(target, steps=2, use_cache=True, verbose=False):
     if use_cache:
         hit = _load_cache().get(key)
         if hit:
-            tile = hit["tile"] if hit["tile"] == FIT else tuple(hit["tile"])
+            tile = hit["tile"] if hit["tile"] in (FIT, FIT_PAIR) else tuple(hit["tile"])
             best = (tile, hit["blocks_per_cu"], hit["cell_waves"])
             apply(target, *best)
             return best, {}
@@ -170,7 +183,7 @@ def tune(target, steps=2, use_cache=True, verbose=False):
         raise RuntimeError("autotune: no candidate schedule fits")
     best, _ = min(times.items(), key=lambda kv: kv[1])
     apply(target, *best)
-    _store_cache(key, {"tile": best[0] if best[0] == FIT else list(best[0]),
+    _store_cache(key, {"tile": best[0] if best[0] in (FIT, FIT_PAIR) else list(best[0]),
                        "blocks_per_cu": best[1], "cell_waves": best[2],
                        "ms": {"%s/%d/%d" % c: v for c, v in times.items()}})
     return best, times

@@ -109,7 +109,7 @@ struct smp_problem {
     int check_every = 64;         // steps between non-finite scans in smp_step (0: none)
     int64_t steps_unchecked = 0;
     void *norm_buf = nullptr;   // partials and results of smp_moment_norms
-    void *sink = nullptr;         // 512 elements: stores of lanes without a target cell
+    void *sink = nullptr;         // 1024 elements: stores of lanes without a target cell
     std::vector<hipEvent_t> events;
     std::vector<void *> ckpt;     // device copies of the whole state
     // discrete adjoint: lambda, mu (state layout), dJ/dsigma per component
@@ -261,7 +261,7 @@ namespace {
 int make_plan(smp_problem *h, int kind, int ncf_reserve) {
     std::string err = "no schedule";
     const size_t slot_bytes = (size_t)h->ts.slot_elems * h->elem;
-    const size_t fixed = smp::substep_lds_overhead() + (size_t)ncf_reserve * 256 * h->elem;
+    const size_t fixed = smp::substep_lds_overhead() + (size_t)ncf_reserve * h->ts.threads * h->elem;
     const int max_load_elems = smp::MAXL * h->ts.slot_elems;
     smp::Plan plan;
     int blocks = h->blocks_per_cu;
@@ -530,6 +530,9 @@ int smp_create(const smp_desc *d, smp_problem **out) {
     if (const char *env = std::getenv("SMP_CELL_WAVES")) cw = std::atoi(env);
     if (d->cell_waves) cw = d->cell_waves;
     if (cw != 4 && cw != 8) return fail(SMP_E_INVALID, "cell_waves must be 4 or 8");
+    if (h->ts.pair && h->precision != SMP_FP32)
+        return fail(SMP_E_INVALID, "tiles of more than 256 cells (two cells per thread) need the "
+                                   "FP32 state");
     h->cell_waves = cw;
     if (want_blocks < 1 || want_blocks > 8) return fail(SMP_E_INVALID, "blocks_per_cu must be 1..8");
     if (const char *env = std::getenv("SMP_CHECK_EVERY")) h->check_every = std::max(0, std::atoi(env));
@@ -548,7 +551,7 @@ int smp_create(const smp_desc *d, smp_problem **out) {
     HIP_TRY(hipMemsetAsync(h->U, 0, (size_t)h->slab * h->elem * d->n_sys, h->stream));
     rc = h->dalloc(&h->red_buf, sizeof(double) * (3 * 1024 + 4));
     if (rc) return rc;
-    rc = h->dalloc(&h->sink, sizeof(double) * 512);
+    rc = h->dalloc(&h->sink, sizeof(double) * 1024);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(h->stream));
     *out = h.release();
@@ -836,7 +839,7 @@ int refresh_coefs(smp_problem *h) {
         // kernel's LDS (blob ring, staged coefficient fields, source slots)
         // and of each component in the state
         const int32_t slots_base =
-            (int32_t)(smp::substep_lds_overhead() + (size_t)h->ncf[kind] * 256 * h->elem);
+            (int32_t)(smp::substep_lds_overhead() + (size_t)h->ncf[kind] * h->ts.threads * h->elem);
         for (smp::TgtRec &r : recs) {
             r.uoff = (int64_t)r.comp * h->slab * h->elem;
             for (int q = 0; q < smp::MAXT; ++q) {
@@ -909,6 +912,7 @@ smp::SubstepArgs<T> make_args(smp_problem *h, int kind, double dt, int napp, int
         a.lds[e] = h->ts.lds[e];
     }
     a.slot_elems = h->ts.slot_elems;
+    a.pair = h->ts.pair ? 1 : 0;
     a.lds0_shift = 0;
     while ((1 << a.lds0_shift) < h->ts.lds[0]) ++a.lds0_shift;
     a.inv_lds1 = 1.0f / (float)h->ts.lds[1];
@@ -1577,8 +1581,12 @@ int batch_run(smp_batch *b, const double *dt, const int64_t *ns, int use_graph) 
     int ncu = 0, per_cu = 0;
     HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device));
     int per_cu1 = 0;
-    HIP_TRY(smp::batch_occupancy<T>(nt, gen, cw, 2, lds, &per_cu));
-    HIP_TRY(smp::batch_occupancy<T>(nt, gen, cw, 1, lds, &per_cu1));
+    const bool pair = b->hs[0]->ts.pair;
+    for (int i = 1; i < n; ++i)
+        if (b->hs[i]->ts.pair != pair)
+            return fail(SMP_E_INVALID, "batched handles must all or none use tiles of two cells per thread");
+    HIP_TRY(smp::batch_occupancy<T>(nt, gen, cw, 2, pair, lds, &per_cu));
+    HIP_TRY(smp::batch_occupancy<T>(nt, gen, cw, 1, pair, lds, &per_cu1));
     per_cu = std::min(per_cu, per_cu1);
     if (per_cu < 1) return fail(SMP_E_INVALID, "the batched kernel does not fit on a CU");
     const int grid = ncu * per_cu;
@@ -1589,7 +1597,7 @@ int batch_run(smp_batch *b, const double *dt, const int64_t *ns, int use_graph) 
         HIP_TRY(smp::launch_fill(b->counters, n_launch, 0, b->stream));
         for (int L = 0; L < n_launch; ++L) {
             if (timed) HIP_TRY(hipEventRecord(b->events[2 + 2 * L], b->stream));
-            HIP_TRY(smp::launch_batch<T>(dl + L, nt, gen, cw, lnapp[L], grid, lds, b->stream));
+            HIP_TRY(smp::launch_batch<T>(dl + L, nt, gen, cw, lnapp[L], pair, grid, lds, b->stream));
             if (timed) HIP_TRY(hipEventRecord(b->events[3 + 2 * L], b->stream));
         }
         return SMP_OK;
@@ -1927,6 +1935,9 @@ int smp_set_schedule(smp_problem *h, const int32_t *tile, int32_t blocks_per_cu,
     } else {
         ts = smp::choose_tile(h->geo, h->elem);
     }
+    if (ts.pair && h->precision != SMP_FP32)
+        return fail(SMP_E_INVALID, "tiles of more than 256 cells (two cells per thread) need the "
+                                   "FP32 state");
     if ((ts.slot_elems * (h->elem / 4) + 63) / 64 > smp::CPS_MAX)
         return fail(SMP_E_INVALID, "a tile with its halo exceeds 4 KB per component");
     const smp::TileShape old_ts = h->ts;

@@ -34,7 +34,7 @@ struct SubstepArgs {
     T *U;                     // target values read
     T *out;                   // target values written (U for an update in place)
     const T *src;             // source components (U for an update in place)
-    T *sink;                  // 256 elements written by lanes without a target cell
+    T *sink;                  // 1024 elements written by lanes without a target cell
     int64_t slab;             // elements per component
     int32_t pitch0, ext1;     // storage index = i0 + pitch0*(i1 + ext1*i2)
     int32_t n[3], bc[3];      // per kernel axis
@@ -52,6 +52,7 @@ struct SubstepArgs {
     double half_dt;
     int32_t napp;             // half-step applications with the same flux (1 or 2)
     int32_t fslot[2];         // filter slot of application 0 and 1
+    int32_t pair;             // two cells per cell thread (tiles of more than 256 cells)
 };
 
 // One problem's substep in a batched launch: its arguments and the index of
@@ -79,7 +80,7 @@ size_t substep_lds_overhead();
 // nt: terms per target the kernel unrolls (8 or 12); gen: some coefficient
 // is a field read from global memory (more fields than the LDS stages, or a
 // spatially varying filter); cw: cell waves (4, or 8 with two waves per
-// SIMD on every cell)
+// SIMD on every cell); a.pair: two cells per cell thread (FP32 state)
 template <typename T>
 hipError_t launch_substep(const SubstepArgs<T> &a, int nt, bool gen, int cw, size_t lds_bytes,
                           hipStream_t s);
@@ -87,13 +88,15 @@ hipError_t launch_substep(const SubstepArgs<T> &a, int nt, bool gen, int cw, siz
 // Persistent batched substep launch over `grid` workgroups; bl is a device
 // pointer.  nt / gen as for launch_substep, over all entries.
 // napp: the half-step applications of every entry of the launch (1 or 2)
+// pair: two cells per cell thread (every entry of the launch)
 template <typename T>
-hipError_t launch_batch(const BatchLaunch<T> *bl, int nt, bool gen, int cw, int napp, int grid,
-                        size_t lds_bytes, hipStream_t s);
+hipError_t launch_batch(const BatchLaunch<T> *bl, int nt, bool gen, int cw, int napp, bool pair,
+                        int grid, size_t lds_bytes, hipStream_t s);
 
 // workgroups of the batched kernel resident per CU with lds_bytes of LDS
 template <typename T>
-hipError_t batch_occupancy(int nt, bool gen, int cw, int napp, size_t lds_bytes, int *blocks);
+hipError_t batch_occupancy(int nt, bool gen, int cw, int napp, bool pair, size_t lds_bytes,
+                           int *blocks);
 
 // p[0..n) = v (a kernel, so that it can be captured in a graph)
 hipError_t launch_fill(int *p, int n, int v, hipStream_t s);

@@ -79,13 +79,22 @@ __device__ __forceinline__ SMP_GLOBAL P *glb(P *p) {
 // space (an entry of the batched kernel, read with scalar loads).
 // CW cell waves (4 or 8): with 8, two waves per SIMD share each cell, each
 // updating half of a group's targets.
-template <typename T, int NT, int NAPP, bool GEN, int CW, int NCW, bool WARM, typename AT>
+//
+// PAIR: every cell thread updates two cells, neighbours along the first
+// kernel axis (tiles of up to 512 cells; with 8 cell waves, two threads share
+// each pair of cells, as they share a cell in the one-cell path).  A term's samples of
+// both cells are adjacent in LDS, so each term costs one address computation
+// and two LDS reads of neighbouring elements for the two cells, and a target
+// record read serves both; each cell's arithmetic is that of the one-cell
+// path, so the results agree bit for bit.
+template <typename T, int NT, int NAPP, bool GEN, int CW, int NCW, bool WARM, bool PAIR, typename AT>
 __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsigned char *smem_raw) {
     constexpr int NCT = CW * 64;              // cell threads
     constexpr int TPT = GMAX * 4 / CW;        // targets of a group per cell thread
+    constexpr int PC = PAIR ? 2 : 1;          // cells per cell thread
     unsigned char *ring = smem_raw;
     T *cfa = reinterpret_cast<T *>(smem_raw + 3 * BLOB_SZ);
-    T *lds = cfa + a.ncf * 256;
+    T *lds = cfa + a.ncf * 256 * PC;
     const int tid = threadIdx.x;
     // arguments the update reads for every target, held in registers (a
     // scalar reload inside the loop would also wait for the LDS reads in
@@ -119,8 +128,17 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     // cell reads the address of the nearest cell (a broadcast, in the slot).
     const bool by_slot = a.lds[0] * a.tile[1] * a.tile[2] <= 256;
     int c0, c1, c2;
-    bool has_cell;
-    if (by_slot) {
+    bool has_cell, has_cell1 = false;
+    if (PAIR) {
+        // slot columns 2p and 2p+1 of row r (the host checks that
+        // lds[0]/2 * tile[1] * tile[2] <= 256); column lds[0]-1 is the halo
+        const int r = ctid >> (a.lds0_shift - 1);
+        c0 = (ctid & ((a.lds[0] >> 1) - 1)) << 1;
+        c1 = r % a.tile[1];
+        c2 = r / a.tile[1];
+        has_cell = tid < NCT && c0 < a.tile[0] && c2 < a.tile[2];
+        has_cell1 = has_cell && c0 + 1 < a.tile[0];
+    } else if (by_slot) {
         const int r = ctid >> a.lds0_shift;
         c0 = ctid & (a.lds[0] - 1);
         c1 = r % a.tile[1];
@@ -134,6 +152,7 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     }
     const int g0 = o0 + c0, g1 = o1 + c1, g2 = o2 + c2;
     const bool in_store = has_cell && g0 < a.E[0] && g1 < a.E[1] && g2 < a.E[2];
+    const bool in_store1 = has_cell1 && g0 + 1 < a.E[0] && g1 < a.E[1] && g2 < a.E[2];
     const int lbase = min(c0, a.tile[0] - 1) + a.lds[0] * (min(c1, a.tile[1] - 1) + a.lds[1] * min(c2, a.tile[2] - 1));
     const int64_t gidx = g0 + (int64_t)a.pitch0 * (g1 + (int64_t)a.ext1 * g2);
     const int S = a.slot_elems;
@@ -257,6 +276,9 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     // draining every load and store in flight.
     SMP_GLOBAL const char *ug_ld = glb(reinterpret_cast<const char *>(a.U)) + (in_store ? gidx : 0) * sizeof(T);
     SMP_GLOBAL char *ug_st = glb(reinterpret_cast<char *>(a.out)) + gidx * sizeof(T);
+    // the second cell of a pair reads its neighbour's address when it is
+    // outside the storage (the value is not used)
+    const int d1 = in_store1 ? (int)sizeof(T) : 0;
     SMP_GLOBAL T *sink = glb(a.sink);
     // The cell waves' records of group gq: with SMP_SCALAR_RECORDS, read
     // with scalar loads from the plan in global memory (every workgroup
@@ -277,7 +299,10 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     auto load_targets = [&](int gq, int buf, T *u) {
         const auto r = rec_of(gq, buf);
 #pragma unroll
-        for (int i = 0; i < TPT; ++i) u[i] = *reinterpret_cast<SMP_GLOBAL const T *>(ug_ld + r[i].uoff);
+        for (int i = 0; i < TPT; ++i) {
+            u[PC * i] = *reinterpret_cast<SMP_GLOBAL const T *>(ug_ld + r[i].uoff);
+            if (PAIR) u[PC * i + 1] = *reinterpret_cast<SMP_GLOBAL const T *>(ug_ld + r[i].uoff + d1);
+        }
     };
     // byte address of this thread's cell in the first source slot; term
     // offsets are byte offsets from the start of LDS
@@ -295,10 +320,11 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     typedef double D;
     // coefficient: staged field (off >= 0), scalar (off == -1) or, in the
     // GEN variant only, a field read from global memory (off == -2)
-    auto coef = [&](int off, const void *p, double s) -> D {
-        const D staged = (D)cfa[(off > 0 ? off : 0) + ctid];
+    auto coef = [&](int off, const void *p, double s, int k) -> D {
+        const D staged = (D)cfa[(off > 0 ? off : 0) + PC * ctid + k];
         D v = off >= 0 ? staged : s;
-        if (GEN && off == -2) v = in_store ? (D)glb(reinterpret_cast<const T *>(p))[gidx] : 0.0;
+        if (GEN && off == -2)
+            v = (k ? in_store1 : in_store) ? (D)glb(reinterpret_cast<const T *>(p))[gidx + k] : 0.0;
         return v;
     };
     // Every group holds GMAX target records; unused ones are zero records
@@ -310,37 +336,47 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
         for (int i = 0; i < TPT; ++i) {
             const auto &r = rb[i];
             const bool act = has_cell & (g0 < r.ext[0]) & (g1 < r.ext[1]) & (g2 < r.ext[2]);
+            const bool act1 = has_cell1 & (g0 + 1 < r.ext[0]) & (g1 < r.ext[1]) & (g2 < r.ext[2]);
             // W = -sumcell(...): the coefficients carry -M(j,k)/h; padding
             // terms add coef 0 times a difference within the zero tile
-            A w = 0;
+            A w[PC];
+#pragma unroll
+            for (int k = 0; k < PC; ++k) w[k] = 0;
 #pragma unroll
             for (int q = 0; q < NT; ++q) {
                 const int toff = r.t[q].off, tstride = r.t[q].stride;
                 const unsigned char *lo = smem_raw + (toff + lb);
                 const A c = sizeof(T) == 4 ? (A)r.t[q].coef32[0] : (A)r.t[q].coef;
-                w += c * ((A)*reinterpret_cast<const T *>(lo + tstride) -
-                          (A)*reinterpret_cast<const T *>(lo));
-            }
-            const D sig = coef(r.sig_off, r.sig_p, r.sig_s);
-            const D E = coef(r.E_off, r.E_p, r.E_s);
-            const D wq = (D)w + coef(r.Q_off, r.Q_p, 0.0);
-            D u = (D)u_in[i];
 #pragma unroll
-            for (int app = 0; app < NAPP; ++app) {
-                // U + dt/2*(W+Q - s.*U).*E   (starmap_solver.m:352-354)
-                u = u + (half_dt * (wq - sig * u)) * E;
-                const bool s1 = (app == 0 ? fslot0 : fslot1) != 0;
-                D f = s1 ? r.F_s[1] : r.F_s[0];   // 1 for targets without a filter
-                if (GEN && r.filt) {
-                    const void *fp = s1 ? r.F[1] : r.F[0];
-                    if (fp && act) f = (D)glb(reinterpret_cast<const T *>(fp))[gidx];
-                }
-                // the state after every half-step is a value of the state
-                // precision, as if stored: fused and separate launches agree
-                u = (D)(T)(u * f);
+                for (int k = 0; k < PC; ++k)
+                    w[k] += c * ((A)*reinterpret_cast<const T *>(lo + tstride + k * sizeof(T)) -
+                                 (A)*reinterpret_cast<const T *>(lo + k * sizeof(T)));
             }
-            SMP_GLOBAL T *dst = act ? reinterpret_cast<SMP_GLOBAL T *>(ug_st + r.uoff) : sink + tid;
-            *dst = (T)u;
+#pragma unroll
+            for (int k = 0; k < PC; ++k) {
+                const bool ak = k ? act1 : act;
+                const D sig = coef(r.sig_off, r.sig_p, r.sig_s, k);
+                const D E = coef(r.E_off, r.E_p, r.E_s, k);
+                const D wq = (D)w[k] + coef(r.Q_off, r.Q_p, 0.0, k);
+                D u = (D)u_in[PC * i + k];
+#pragma unroll
+                for (int app = 0; app < NAPP; ++app) {
+                    // U + dt/2*(W+Q - s.*U).*E   (starmap_solver.m:352-354)
+                    u = u + (half_dt * (wq - sig * u)) * E;
+                    const bool s1 = (app == 0 ? fslot0 : fslot1) != 0;
+                    D f = s1 ? r.F_s[1] : r.F_s[0];   // 1 for targets without a filter
+                    if (GEN && r.filt) {
+                        const void *fp = s1 ? r.F[1] : r.F[0];
+                        if (fp && ak) f = (D)glb(reinterpret_cast<const T *>(fp))[gidx + k];
+                    }
+                    // the state after every half-step is a value of the state
+                    // precision, as if stored: fused and separate launches agree
+                    u = (D)(T)(u * f);
+                }
+                SMP_GLOBAL T *dst = ak ? reinterpret_cast<SMP_GLOBAL T *>(ug_st + r.uoff + k * sizeof(T))
+                                       : sink + PC * tid + k;
+                *dst = (T)u;
+            }
         }
     };
 
@@ -391,17 +427,23 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     // by two, so no values move between sets): they arrive while the copy
     // waves finish, and the next phase's first read of them is the only wait
     // for memory in the cell waves
-    T uA[TPT], uB[TPT];
+    T uA[TPT * PC], uB[TPT * PC];
     if (half == 0) {
         // every field load in flight before the first LDS write
-        T cv[MAX_CFIELDS];
+        T cv[MAX_CFIELDS * PC];
 #pragma unroll
         for (int c = 0; c < MAX_CFIELDS; ++c)
-            if (c < a.ncf)
-                cv[c] = glb(reinterpret_cast<const T *>(a.cfields[c]))[in_store ? gidx : 0];
+            if (c < a.ncf) {
+                const T *fc = reinterpret_cast<const T *>(a.cfields[c]);
+                cv[PC * c] = glb(fc)[in_store ? gidx : 0];
+                if (PAIR) cv[PC * c + 1] = glb(fc)[in_store1 ? gidx + 1 : 0];
+            }
 #pragma unroll
         for (int c = 0; c < MAX_CFIELDS; ++c)
-            if (c < a.ncf) cfa[c * 256 + ctid] = in_store ? cv[c] : (T)0;
+            if (c < a.ncf) {
+                cfa[c * 256 * PC + PC * ctid] = in_store ? cv[PC * c] : (T)0;
+                if (PAIR) cfa[c * 256 * PC + PC * ctid + 1] = in_store1 ? cv[PC * c + 1] : (T)0;
+            }
     }
     for (int e = tid; e < S; e += NCT) lds[e] = (T)0;   // slot 0: the zero tile
     __syncthreads();
@@ -418,19 +460,19 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     }
 }
 
-template <typename T, int NT, int NAPP, bool GEN, int CW>
+template <typename T, int NT, int NAPP, bool GEN, int CW, bool PAIR>
 __global__ __launch_bounds__(kthreads(CW)) void substep_kernel(const SubstepArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= a.n_tiles) return;
-    substep_tile<T, NT, NAPP, GEN, CW, copy_waves(CW), false>(a, tile, smem_raw);
+    substep_tile<T, NT, NAPP, GEN, CW, copy_waves(CW), false, PAIR>(a, tile, smem_raw);
 }
 
 // Batched persistent kernel: one launch runs a substep of many independent
 // problems.  Work units are (entry, tile) pairs, entries ordered by cost per
 // tile, heaviest first; every workgroup takes the next unit from a counter
 // until none is left, so uneven problems balance over the CUs.
-template <typename T, int NT, int NAPP, bool GEN, int CW>
+template <typename T, int NT, int NAPP, bool GEN, int CW, bool PAIR>
 __global__ __launch_bounds__(batch_threads(CW)) void batch_kernel(const BatchLaunch<T> *bl) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int s_unit;
@@ -468,7 +510,8 @@ __global__ __launch_bounds__(batch_threads(CW)) void batch_kernel(const BatchLau
         // the launch's application count is compiled in (a runtime count
         // costs the update its instruction schedule, and both tile paths in
         // one kernel outgrow the instruction cache)
-        substep_tile<T, NT, NAPP, GEN, CW, batch_copy_waves(CW), true>(a, u - ent[lo].unit0, smem_raw);
+        substep_tile<T, NT, NAPP, GEN, CW, batch_copy_waves(CW), true, PAIR>(a, u - ent[lo].unit0,
+                                                                               smem_raw);
     }
 }
 
@@ -477,74 +520,104 @@ __global__ void fill_kernel(int *p, int n, int v) {
     if (i < n) p[i] = v;
 }
 
-template <typename T, int NT, int NAPP, bool GEN, int CW>
+template <typename T, int NT, int NAPP, bool GEN, int CW, bool PAIR>
 static hipError_t launch_v(const SubstepArgs<T> &a, size_t lds_bytes, hipStream_t s) {
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void *)substep_kernel<T, NT, NAPP, GEN, CW>,
+        hipError_t e = hipFuncSetAttribute((const void *)substep_kernel<T, NT, NAPP, GEN, CW, PAIR>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            160 * 1024);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
     const int nb = ((a.n_tiles + 7) / 8) * 8;
-    hipLaunchKernelGGL((substep_kernel<T, NT, NAPP, GEN, CW>), dim3(nb), dim3(kthreads(CW)),
+    hipLaunchKernelGGL((substep_kernel<T, NT, NAPP, GEN, CW, PAIR>), dim3(nb), dim3(kthreads(CW)),
                        lds_bytes, s, a);
     return hipGetLastError();
 }
 
-template <typename T, int NT, int CW>
+template <typename T, int NT, int CW, bool PAIR>
 static hipError_t launch_nt(const SubstepArgs<T> &a, bool gen, size_t lds_bytes, hipStream_t s) {
     if (a.napp == 1)
-        return gen ? launch_v<T, NT, 1, true, CW>(a, lds_bytes, s)
-                   : launch_v<T, NT, 1, false, CW>(a, lds_bytes, s);
-    return gen ? launch_v<T, NT, 2, true, CW>(a, lds_bytes, s)
-               : launch_v<T, NT, 2, false, CW>(a, lds_bytes, s);
+        return gen ? launch_v<T, NT, 1, true, CW, PAIR>(a, lds_bytes, s)
+                   : launch_v<T, NT, 1, false, CW, PAIR>(a, lds_bytes, s);
+    return gen ? launch_v<T, NT, 2, true, CW, PAIR>(a, lds_bytes, s)
+               : launch_v<T, NT, 2, false, CW, PAIR>(a, lds_bytes, s);
+}
+
+// Two cells per thread: FP32 state (the only variants built)
+template <typename T>
+static constexpr bool pair_built(int cw) {
+    return sizeof(T) == 4 && (cw == 4 || cw == 8);
 }
 
 template <typename T>
 hipError_t launch_substep(const SubstepArgs<T> &a, int nt, bool gen, int cw, size_t lds_bytes,
                           hipStream_t s) {
     if (a.ncf > MAX_CFIELDS || (cw != 4 && cw != 8) || a.chunks > CPS_MAX ||
-        (a.napp != 1 && a.napp != 2))
+        (a.napp != 1 && a.napp != 2) || (a.pair && !pair_built<T>(cw)))
         return hipErrorInvalidValue;
-    if (cw == 8) return nt <= 8 ? launch_nt<T, 8, 8>(a, gen, lds_bytes, s)
-                                : launch_nt<T, MAXT, 8>(a, gen, lds_bytes, s);
-    return nt <= 8 ? launch_nt<T, 8, 4>(a, gen, lds_bytes, s)
-                   : launch_nt<T, MAXT, 4>(a, gen, lds_bytes, s);
+    if (a.pair) {
+        if constexpr (sizeof(T) == 4) {
+            if (cw == 8)
+                return nt <= 8 ? launch_nt<T, 8, 8, true>(a, gen, lds_bytes, s)
+                               : launch_nt<T, MAXT, 8, true>(a, gen, lds_bytes, s);
+            return nt <= 8 ? launch_nt<T, 8, 4, true>(a, gen, lds_bytes, s)
+                           : launch_nt<T, MAXT, 4, true>(a, gen, lds_bytes, s);
+        }
+        return hipErrorInvalidValue;
+    }
+    if (cw == 8) return nt <= 8 ? launch_nt<T, 8, 8, false>(a, gen, lds_bytes, s)
+                                : launch_nt<T, MAXT, 8, false>(a, gen, lds_bytes, s);
+    return nt <= 8 ? launch_nt<T, 8, 4, false>(a, gen, lds_bytes, s)
+                   : launch_nt<T, MAXT, 4, false>(a, gen, lds_bytes, s);
 }
 
 // (the dynamic LDS limit of batch_kernel is raised by batch_occupancy, which
 // runs before any launch and outside graph capture)
-template <typename T, int NAPP, int CW>
+template <typename T, int NAPP, int CW, bool PAIR>
 static const void *batch_fn_cw(int nt, bool gen) {
     if (nt <= 8)
-        return gen ? (const void *)batch_kernel<T, 8, NAPP, true, CW>
-                   : (const void *)batch_kernel<T, 8, NAPP, false, CW>;
-    return gen ? (const void *)batch_kernel<T, MAXT, NAPP, true, CW>
-               : (const void *)batch_kernel<T, MAXT, NAPP, false, CW>;
+        return gen ? (const void *)batch_kernel<T, 8, NAPP, true, CW, PAIR>
+                   : (const void *)batch_kernel<T, 8, NAPP, false, CW, PAIR>;
+    return gen ? (const void *)batch_kernel<T, MAXT, NAPP, true, CW, PAIR>
+               : (const void *)batch_kernel<T, MAXT, NAPP, false, CW, PAIR>;
 }
 
 template <typename T>
-static const void *batch_fn(int nt, bool gen, int cw, int napp) {
-    if (napp == 2) return cw == 8 ? batch_fn_cw<T, 2, 8>(nt, gen) : batch_fn_cw<T, 2, 4>(nt, gen);
-    return cw == 8 ? batch_fn_cw<T, 1, 8>(nt, gen) : batch_fn_cw<T, 1, 4>(nt, gen);
+static const void *batch_fn(int nt, bool gen, int cw, int napp, bool pair) {
+    if (pair) {
+        if constexpr (sizeof(T) == 4) {
+            if (cw == 8)
+                return napp == 2 ? batch_fn_cw<T, 2, 8, true>(nt, gen) : batch_fn_cw<T, 1, 8, true>(nt, gen);
+            return napp == 2 ? batch_fn_cw<T, 2, 4, true>(nt, gen) : batch_fn_cw<T, 1, 4, true>(nt, gen);
+        }
+        return nullptr;
+    }
+    if (napp == 2)
+        return cw == 8 ? batch_fn_cw<T, 2, 8, false>(nt, gen) : batch_fn_cw<T, 2, 4, false>(nt, gen);
+    return cw == 8 ? batch_fn_cw<T, 1, 8, false>(nt, gen) : batch_fn_cw<T, 1, 4, false>(nt, gen);
 }
 
 template <typename T>
-hipError_t launch_batch(const BatchLaunch<T> *bl, int nt, bool gen, int cw, int napp, int grid,
-                        size_t lds_bytes, hipStream_t s) {
-    if (grid < 1 || lds_bytes > 160 * 1024 - 64 || (cw != 4 && cw != 8) || (napp != 1 && napp != 2))
+hipError_t launch_batch(const BatchLaunch<T> *bl, int nt, bool gen, int cw, int napp, bool pair,
+                        int grid, size_t lds_bytes, hipStream_t s) {
+    if (grid < 1 || lds_bytes > 160 * 1024 - 64 || (cw != 4 && cw != 8) || (napp != 1 && napp != 2) ||
+        (pair && !pair_built<T>(cw)))
         return hipErrorInvalidValue;
-    const void *f = batch_fn<T>(nt, gen, cw, napp);
+    const void *f = batch_fn<T>(nt, gen, cw, napp, pair);
+    if (!f) return hipErrorInvalidValue;
     void *args[] = {(void *)&bl};
     return hipLaunchKernel(f, dim3(grid), dim3(batch_threads(cw)), args, lds_bytes, s);
 }
 
 template <typename T>
-hipError_t batch_occupancy(int nt, bool gen, int cw, int napp, size_t lds_bytes, int *blocks) {
-    if ((cw != 4 && cw != 8) || (napp != 1 && napp != 2)) return hipErrorInvalidValue;
-    const void *f = batch_fn<T>(nt, gen, cw, napp);
+hipError_t batch_occupancy(int nt, bool gen, int cw, int napp, bool pair, size_t lds_bytes,
+                           int *blocks) {
+    if ((cw != 4 && cw != 8) || (napp != 1 && napp != 2) || (pair && !pair_built<T>(cw)))
+        return hipErrorInvalidValue;
+    const void *f = batch_fn<T>(nt, gen, cw, napp, pair);
+    if (!f) return hipErrorInvalidValue;
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64);
     if (e != hipSuccess) return e;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, f, batch_threads(cw), lds_bytes);
@@ -710,9 +783,9 @@ hipError_t launch_reduce_final(const double *part, int blocks, double *out3, hip
                                          hipStream_t);                                         \
     template hipError_t launch_reduce<T>(const T *, const ReduceGeom &, double *, int,         \
                                          hipStream_t);                                         \
-    template hipError_t launch_batch<T>(const BatchLaunch<T> *, int, bool, int, int, int, size_t, \
-                                        hipStream_t);                                          \
-    template hipError_t batch_occupancy<T>(int, bool, int, int, size_t, int *);
+    template hipError_t launch_batch<T>(const BatchLaunch<T> *, int, bool, int, int, bool, int,  \
+                                        size_t, hipStream_t);                                  \
+    template hipError_t batch_occupancy<T>(int, bool, int, int, bool, size_t, int *);
 
 SMP_INSTANTIATE(double)
 SMP_INSTANTIATE(float)

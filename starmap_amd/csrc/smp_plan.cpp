@@ -92,10 +92,15 @@ std::string make_tile(const Geometry &g, int tx, int ty, int tz, TileShape &ts) 
         ts.tile[e] = e < g.ndim ? t[e] : 1;
         if (ts.tile[e] < 1) return "tile extents must be positive";
     }
-    if (ts.cells() > 256) return "a tile holds at most 256 cells";
     // LDS rows of a power of two: the tile plus its one-cell halo
     if ((ts.tile[0] + 1) & ts.tile[0]) return "tile[0]+1 must be a power of two";
     ts.threads = 256;
+    if (ts.cells() > 256) {
+        if (ts.cells() > 512 || ts.tile[0] < 3 || ((ts.tile[0] + 1) / 2) * ts.tile[1] * ts.tile[2] > 256)
+            return "a tile holds at most 256 cells, or 512 with (tile[0]+1)/2*tile[1]*tile[2] <= 256";
+        ts.pair = true;
+        ts.threads = 512;
+    }
     for (int e = 0; e < 3; ++e) ts.lds[e] = e < g.ndim ? ts.tile[e] + 1 : 1;
     ts.slot_elems = ts.lds[0] * ts.lds[1] * ts.lds[2];
     return "";

@@ -91,11 +91,15 @@ struct TileShape {
     int tile[3] = {1, 1, 1};     // cells per tile per kernel axis
     int lds[3] = {1, 1, 1};      // tile + 1 halo per axis
     int slot_elems = 1;
-    int threads = 256;
+    int threads = 256;           // cells a tile's staged coefficient fields hold
+    bool pair = false;           // two cells per cell thread
     int cells() const { return tile[0] * tile[1] * tile[2]; }
 };
 
-// tile of tx*ty*tz <= 256 cells (256 threads, one cell each); "" or an error
+// tile of tx*ty*tz <= 256 cells (256 threads, one cell each), or of up to
+// 512 cells with two neighbours along the first axis per thread (pair: the
+// slot's rows of lds[0] = tx+1 elements give lds[0]/2 threads each, at most
+// 256 in all); "" or an error
 std::string make_tile(const Geometry &g, int tx, int ty, int tz, TileShape &ts);
 TileShape choose_tile(const Geometry &g, int elem_bytes);
 

@@ -143,7 +143,7 @@ def test_batch_and_schedule_host_checks():
         assert s.get_schedule() == ((127, 2, 1), 2, 8)
         with pytest.raises(SmpError, match="power of two"):
             s.set_schedule((30, 4), 1)
-        with pytest.raises(SmpError, match="at most 256 cells"):
+        with pytest.raises(SmpError, match="more than 256 cells"):
             s.set_schedule((63, 8), 1)
         assert s.get_schedule() == ((127, 2, 1), 2, 8)
         with pytest.raises(SmpError, match="cell_waves"):
@@ -223,3 +223,27 @@ def test_matlab_gateway_matches_header_and_wrapper():
     commands = set(re.findall(r'strcmp\(cmd, "([a-z_0-9]+)"\)', c_src))
     sent = set(re.findall(r"starmap_hip_mex\('([a-z_0-9]+)'", m_src))
     assert sent and sent <= commands, sent - commands
+
+
+def test_pair_tiles_need_fp32():
+    """Tiles of more than 256 cells (two cells per thread) are accepted for
+    the FP32 state and refused for FP64."""
+    from starmap_amd import problems as P
+    from starmap_amd.solver import HipSolver
+    par = I.starmap_init(P.lattice_2d(n=40, n_mom=5))
+    s = HipSolver(par, device=-1, precision="fp32", cell_waves=4)
+    try:
+        s.set_schedule((63, 8), 2, 4)
+        assert s.get_schedule() == ((63, 8, 1), 2, 4)
+        s.set_schedule((127, 4), 1, 8)
+        assert s.get_schedule() == ((127, 4, 1), 1, 8)
+        with pytest.raises(_lib.SmpError, match="at most 256 cells"):
+            s.set_schedule((63, 9), 2, 4)
+    finally:
+        s.close()
+    s = HipSolver(par, device=-1, precision="fp64", cell_waves=4)
+    try:
+        with pytest.raises(_lib.SmpError, match="FP32"):
+            s.set_schedule((63, 8), 2, 4)
+    finally:
+        s.close()

@@ -186,3 +186,57 @@ def test_batch_mixed_tiles_bitwise(precision):
                 assert np.array_equal(x, y), (i, j)
     finally:
         b.close()
+
+
+def _spatial_filter(par, dt, k, x, y):
+    # a filter field (varies in space): the kernel reads it from global memory
+    return np.exp(-dt * 8.0 * (1 + 0.1 * x) * (k / par["n_mom"]) ** 4)
+
+
+@pytest.mark.parametrize("case", ["lattice2d", "lattice2d_filter_field", "gauss3d"])
+def test_pair_tiles_keep_every_bit(case):
+    """FP32 tiles of two cells per thread (up to 512 cells) give the same
+    bits as the one-cell tiles: every cell's arithmetic is unchanged."""
+    if case == "lattice2d":
+        prob, tiles = P.lattice_2d(n=70, n_mom=9, tfinal=0.3), [(63, 8), (127, 4), (31, 16), (15, 32)]
+    elif case == "lattice2d_filter_field":
+        prob = dict(P.lattice_2d(n=50, n_mom=7, tfinal=0.3), filterfunction=_spatial_filter)
+        tiles = [(63, 8), (31, 16)]
+    else:
+        prob = P.gauss_periodic_3d(n=18, n_mom=5, tfinal=0.3, std=0.2)
+        tiles = [(31, 4, 4), (15, 8, 4)]
+    par = I.starmap_init(prob)
+    res = []
+    for tile, bpc, cw in [(None, 0, 4)] + [(t, b, c) for t in tiles for b, c in ((1, 8), (2, 4))]:
+        print("pair", case, tile, bpc, cw, flush=True)
+        s = HipSolver(par, precision="fp32", cell_waves=4)
+        try:
+            dt = s.prepare()
+            s.step(dt, 2)
+            if tile is not None:
+                s.set_schedule(tile, bpc, cw)
+            s.step(dt, 5)
+            res.append(states(s))
+        finally:
+            s.close()
+    for other in res[1:]:
+        for x, y in zip(res[0], other):
+            assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("use_graph,bpc,cw", [(False, 2, 4), (True, 2, 4), (True, 1, 8)])
+def test_batch_with_pair_tiles_bitwise_equal_single_handles(use_graph, bpc, cw):
+    pars = small_sweep()
+    ref = single_runs(pars, "fp32")
+    b = BatchSolver(pars, precision="fp32", cell_waves=cw)
+    try:
+        autotune.apply(b, autotune.FIT_PAIR, bpc, cw)
+        assert all(int(np.prod(s.get_schedule()[0])) > 256 or max(s.n[1:]) < 64
+                   for s in b.solvers)
+        dts, ns = b.prepare()
+        b.step(ns, use_graph=use_graph)
+        for i, s in enumerate(b.solvers):
+            for j, (x, y) in enumerate(zip(states(s), ref[i])):
+                assert np.array_equal(x, y), (i, j)
+    finally:
+        b.close()
