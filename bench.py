@@ -250,14 +250,18 @@ def batched_leg(fp64_plain_value, n_instances=64, check=True):
     return leg
 
 
-def adjoint_leg(n=512, n_mom=15, steps=2000, every=50, fd_cells=8, seed=2027):
+def adjoint_leg(n=512, n_mom=15, steps=2000, every=50, fd_cells=64, fd_cells_s=16, seed=2027,
+                memory_frac=0.7):
     """2D P_15 (136 moments) on 512^2, extrapolation BC, per-cell
     sigma_a = 0.5*exp(Gaussian random field, correlation length 0.2, std
     0.5), sigma_s0 = 1, Gaussian source (std 0.05) at (-0.5,0), 2000 steps,
-    FP32 state: the forward alone, then the discrete adjoint (checkpoint
-    every 50 steps, FP64 gradient accumulation) of J = h^2 * sum of U1(T)
-    over x > 0.8; per-cell dJ/dsigma_a, dJ/dsigma_s0 against FP64 central
-    differences (relative step 1e-6) at seeded cells."""
+    FP32 state: the forward alone, then the discrete adjoint (segments of 50
+    steps, FP64 gradient accumulation) of J = h^2 * sum of U1(T) over
+    x > 0.8, first with the default memory (a checkpoint per segment and two
+    segments' records), then within a budget of memory_frac of the free
+    device memory (the last segments recorded in the forward pass, the
+    others recomputed); per-cell dJ/dsigma_a and dJ/dsigma_s0 against FP64
+    central differences at seeded cells."""
     from starmap_amd import autotune, init
     from starmap_amd import problems as P
     from starmap_amd.adjoint import Adjoint, objective_value, strip_objective
@@ -278,9 +282,21 @@ def adjoint_leg(n=512, n_mom=15, steps=2000, every=50, fd_cells=8, seed=2027):
         fwd_ms = s.timing()[0]
         J32 = objective_value(s, mask)
         s.run({0: mask}, n_steps=steps, every=every)
-        f_ms, r_ms = s.adjoint_timing()
+        f0_ms, r0_ms = s.adjoint_timing()
+        st0 = s.schedule_stats()
         owner = dict(prob, sigma_a=(lambda x, y: prob["_owner"](x, y).astype(np.float64)))
+        ga0, gs0 = s.cell_gradients(owner, n * n)
+        import torch
+        free = int(torch.cuda.mem_get_info()[0])
+        budget = int(memory_frac * free)
+        log("adjoint: default memory %.1f GB, %.2f s; budget %.1f GB (%.0f%% of %.1f GB free)"
+            % (st0["peak_bytes"] / 1e9, (f0_ms + r0_ms) / 1e3, budget / 1e9, 100 * memory_frac,
+               free / 1e9))
+        s.run({0: mask}, n_steps=steps, every=every, memory=budget)
+        f_ms, r_ms = s.adjoint_timing()
+        st = s.schedule_stats()
         ga, gs = s.cell_gradients(owner, n * n)
+        same = bool(np.array_equal(ga, ga0) and np.array_equal(gs, gs0))
         # algorithmic bytes of one fused update launch (targets of one
         # parity, mean over the two): per target and point U0, lambda
         # read and written, mu written; per source and point its state and
@@ -303,6 +319,11 @@ def adjoint_leg(n=512, n_mom=15, steps=2000, every=50, fd_cells=8, seed=2027):
         "adjoint_reverse_ms": r_ms,
         "cost_vs_forward": (f_ms + r_ms) / fwd_ms,
         "checkpoint_every": every,
+        "memory_budget_bytes": budget,
+        "schedule_stats": st,
+        "default_memory": {"adjoint_forward_ms": f0_ms, "adjoint_reverse_ms": r0_ms,
+                           "cost_vs_forward": (f0_ms + r0_ms) / fwd_ms, "schedule_stats": st0},
+        "gradients_bitwise_equal_across_schedules": same,
         "J": J32,
         "update_alg_bytes_per_launch": upd_bytes,
         "launches_per_step": {"forward": 2, "recompute": 2, "adjoint_update": 2},
@@ -310,12 +331,12 @@ def adjoint_leg(n=512, n_mom=15, steps=2000, every=50, fd_cells=8, seed=2027):
     }
     if fd_cells > 0:
         leg["fd_check"] = adjoint_fd_check(prob, par, mask, n, n_mom, steps, every, fd_cells,
-                                           seed, (ga, gs))
+                                           seed, (ga, gs), fd_cells_s=fd_cells_s)
     return leg
 
 
 def adjoint_fd_check(prob, par, mask, n, n_mom, steps, every, fd_cells, seed, grads32,
-                     rel_step=1e-4):
+                     rel_step=1e-4, fd_cells_s=None):
     """Per-cell dJ/dsigma_a and dJ/dsigma_s0 at seeded cells: the FP32-state
     adjoint and an FP64-state adjoint against FP64 central differences of
     the forward (relative step rel_step: its truncation error is about
@@ -340,9 +361,10 @@ def adjoint_fd_check(prob, par, mask, n, n_mom, steps, every, fd_cells, seed, gr
     ss0 = np.ones((n, n))
     h = HipSolver(par, precision="fp64")
     fd = {"a": [], "s": []}
+    ncell = {"a": fd_cells, "s": fd_cells if fd_cells_s is None else min(fd_cells_s, fd_cells)}
     try:
         for which in ("a", "s"):
-            for c in cells:
+            for c in cells[:ncell[which]]:
                 vals = []
                 base = sa0 if which == "a" else ss0
                 eps = rel_step * base.flat[c]
@@ -356,17 +378,23 @@ def adjoint_fd_check(prob, par, mask, n, n_mom, steps, every, fd_cells, seed, gr
                     h.step(d, steps)
                     vals.append(objective_value(h, mask))
                 fd[which].append((vals[0] - vals[1]) / (2 * eps))
+                if len(fd[which]) % 16 == 0:
+                    log("adjoint: %d finite differences of sigma_%s" % (len(fd[which]), which))
             log("adjoint: finite differences of sigma_%s done" % which)
     finally:
         h.close()
     res = {"cells": [int(c) for c in cells], "rel_step": rel_step, "forward_precision": "fp64"}
     for which, g32, g64 in (("a", grads32[0], ga64), ("s", grads32[1], gs64)):
         f = np.array(fd[which])
-        a32, a64 = g32[cells], g64[cells]
+        cw = cells[:len(f)]
+        a32, a64 = g32[cw], g64[cw]
         res["sigma_" + which] = {
+            "cells": len(f),
             "finite_difference": f.tolist(),
             "adjoint_fp64_state": a64.tolist(),
             "adjoint_fp32_state": a32.tolist(),
+            "rel_err_fp32_adjoint_vs_fd_per_cell": (np.abs(a32 - f) / np.abs(f)).tolist(),
+            "median_rel_err_fp32_adjoint_vs_fd": float(np.median(np.abs(a32 - f) / np.abs(f))),
             "max_rel_err_fp64_adjoint_vs_fd": float(np.max(np.abs(a64 - f) / np.abs(f))),
             "max_rel_err_fp32_adjoint_vs_fd": float(np.max(np.abs(a32 - f) / np.abs(f))),
             "max_rel_diff_fp32_vs_fp64_adjoint": float(np.max(np.abs(a32 - a64) / np.abs(a64))),
@@ -462,8 +490,9 @@ def main():
                     help="skip the FP64 solve of every batched problem")
     ap.add_argument("--adjoint-steps", type=int, default=2000,
                     help="steps of the adjoint leg (0 skips it)")
-    ap.add_argument("--fd-cells", type=int, default=8,
-                    help="cells of the adjoint leg's finite-difference check")
+    ap.add_argument("--fd-cells", type=int, default=64,
+                    help="cells of the adjoint leg's finite-difference check of dJ/dsigma_a "
+                         "(dJ/dsigma_s0 at the first 16 of them)")
     ap.add_argument("--no-tune", action="store_true",
                     help="library default schedules instead of the autotuner's (profiling runs)")
     args = ap.parse_args()

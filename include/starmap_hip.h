@@ -265,6 +265,35 @@ int smp_adjoint_get(smp_problem *h, int32_t what, int32_t j, double *dst, int64_
  * (HIP events on the handle's stream). */
 int smp_adjoint_timing(const smp_problem *h, double *forward_ms, double *reverse_ms);
 
+/* Device memory smp_adjoint may hold for checkpoints and recorded launch
+ * outputs, in bytes (0, the default: a checkpoint at the start of every
+ * segment of checkpoint_every steps and the records of two segments).  Under
+ * a budget the schedule records the last segments during the forward pass
+ * (they are never recomputed) and keeps checkpoints for the others; when
+ * fewer checkpoints fit than there are segments left, it places them by
+ * binomial checkpointing and re-advances segments from them.  A budget
+ * below what one segment needs fails with SMP_E_OOM. */
+int smp_adjoint_set_memory(smp_problem *h, int64_t bytes);
+
+/* The schedule of the last smp_adjoint: stats[0..7] = segments, steps per
+ * segment, segments recorded in the forward pass, checkpoint slots,
+ * segments recomputed, further segment advances from checkpoints,
+ * state-sized buffers and bytes at the peak. */
+int smp_adjoint_stats(const smp_problem *h, int64_t stats[8]);
+
+/* Host-only planner behind smp_adjoint: the schedule of n_steps steps in
+ * segments of `every` steps within `buffers` state-sized buffers (-1: no
+ * budget).  stats as smp_adjoint_stats (bytes 0).  If actions is not null it
+ * receives 6 int32 per action {kind, first segment, end segment, source
+ * buffer, destination buffer, record range}: the forward pass (*n_fwd
+ * actions), then the reverse sweep (*n_actions in all).  Kinds: 0 run in
+ * place, 1 copy the state to a buffer, 2 record a segment, 3 final state from
+ * records, 4 copy and re-advance, 5 recompute into records, 6 reverse.
+ * Buffers: >= 0 checkpoint slots, -2 scratch state, -3 the state at the
+ * start of the recorded segments, -4 the previous segment's records. */
+int smp_adjoint_schedule(int64_t n_steps, int32_t every, int64_t buffers, int64_t stats[8],
+                         int32_t *actions, int64_t cap, int64_t *n_fwd, int64_t *n_actions);
+
 /* Moment norms of the current state, per component j in the reference's
  * output layout: sumsq[j] = sum of U_j^2 over its grid points, diffsq[3*j+e]
  * = sum of squared differences of neighbours along output dimension e+1

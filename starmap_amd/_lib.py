@@ -28,7 +28,8 @@ EXPORTS = ["smp_last_error", "smp_version", "smp_create", "smp_destroy", "smp_gr
            "smp_checkpoint_free", "smp_batch_create", "smp_batch_destroy", "smp_batch_run",
            "smp_batch_timing", "smp_adjoint_reset", "smp_adjoint_seed", "smp_adjoint",
            "smp_adjoint_get", "smp_adjoint_timing", "smp_moment_norms", "smp_probe",
-           "smp_check_finite", "smp_probe_noalloc", "smp_num_components"]
+           "smp_check_finite", "smp_probe_noalloc", "smp_num_components",
+           "smp_adjoint_set_memory", "smp_adjoint_stats", "smp_adjoint_schedule"]
 
 
 class SmpError(RuntimeError):
@@ -107,6 +108,9 @@ def lib():
         "smp_check_finite": (ctypes.c_int, [vp, vp]),
         "smp_probe_noalloc": (ctypes.c_int, [vp, vp, vp, vp]),
         "smp_num_components": (ctypes.c_int, [vp, vp]),
+        "smp_adjoint_set_memory": (ctypes.c_int, [vp, i64]),
+        "smp_adjoint_stats": (ctypes.c_int, [vp, vp]),
+        "smp_adjoint_schedule": (ctypes.c_int, [i64, i32, i64, vp, vp, i64, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

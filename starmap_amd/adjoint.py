@@ -24,6 +24,27 @@ from .batch import count_steps
 from .init import capargs, starmap_init
 from .solver import HipSolver
 
+ACTION_KINDS = ("run", "checkpoint", "record", "final", "advance", "recompute", "reverse")
+
+
+def adjoint_schedule(n_steps, every, buffers):
+    """The host planner behind smp_adjoint: (stats dict, forward actions,
+    reverse actions), each action (kind, first segment, end segment, source
+    buffer, destination buffer, record range); buffers -1: no budget."""
+    L = _lib.lib()
+    st = np.zeros(8, dtype=np.int64)
+    nf, na = ctypes.c_int64(), ctypes.c_int64()
+    _lib.check(L.smp_adjoint_schedule(int(n_steps), int(every), int(buffers), st.ctypes.data,
+                                      None, 0, ctypes.byref(nf), ctypes.byref(na)))
+    acts = np.zeros((max(1, na.value), 6), dtype=np.int32)
+    _lib.check(L.smp_adjoint_schedule(int(n_steps), int(every), int(buffers), st.ctypes.data,
+                                      acts.ctypes.data, acts.shape[0], ctypes.byref(nf),
+                                      ctypes.byref(na)))
+    keys = ("segments", "steps_per_segment", "recorded_in_forward", "checkpoint_slots",
+            "recomputed_segments", "extra_segment_advances", "peak_buffers", "peak_bytes")
+    rows = [(ACTION_KINDS[a[0]],) + tuple(int(v) for v in a[1:]) for a in acts[:na.value]]
+    return dict(zip(keys, (int(v) for v in st))), rows[:nf.value], rows[nf.value:]
+
 
 class Adjoint(HipSolver):
     """A HipSolver with the adjoint entry points."""
@@ -55,13 +76,32 @@ class Adjoint(HipSolver):
         _lib.check(_lib.lib().smp_adjoint_timing(self.handle, ctypes.byref(f), ctypes.byref(r)))
         return f.value, r.value
 
-    def run(self, seeds, n_steps=None, every=50):
+    def set_memory(self, nbytes):
+        """Device memory the adjoint may hold for checkpoints and recorded
+        launch outputs (0: a checkpoint per segment and two segments'
+        records)."""
+        _lib.check(_lib.lib().smp_adjoint_set_memory(self.handle, int(nbytes)))
+
+    def schedule_stats(self):
+        """The last run's schedule: segments, steps per segment, segments
+        recorded in the forward pass, checkpoint slots, segments recomputed,
+        further segment advances, state buffers and bytes at the peak."""
+        st = np.zeros(8, dtype=np.int64)
+        _lib.check(_lib.lib().smp_adjoint_stats(self.handle, st.ctypes.data))
+        keys = ("segments", "steps_per_segment", "recorded_in_forward", "checkpoint_slots",
+                "recomputed_segments", "extra_segment_advances", "peak_buffers", "peak_bytes")
+        return dict(zip(keys, (int(v) for v in st)))
+
+    def run(self, seeds, n_steps=None, every=50, memory=None):
         """Initial state and coefficients (prepare), then the adjoint of
-        n_steps steps (default: to t_final) seeded with {j: dJ/dU_j(T)}.
+        n_steps steps (default: to t_final) seeded with {j: dJ/dU_j(T)},
+        within `memory` bytes of checkpoints and records if given.
         Returns (dt, n_steps)."""
         dt = self.prepare()
         if n_steps is None:
             n_steps = count_steps(dt, self.par["tfinal"])
+        if memory is not None:
+            self.set_memory(memory)
         self.reset()
         for j, arr in seeds.items():
             self.seed(j, arr)

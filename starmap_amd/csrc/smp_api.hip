@@ -125,7 +125,16 @@ struct smp_problem {
     int32_t adj_grp0[2][9] = {};
     int32_t adj_gcode[2][8] = {};
     void *adj_comp = nullptr, *adj_rows = nullptr, *adj_cols = nullptr, *adj_list = nullptr;
-    std::vector<void *> adj_seg, adj_rec;
+    // adjoint schedule buffers: checkpoint slots, record ranges (each of
+    // adj_range_rb state-sized buffers, two launches' records per buffer),
+    // the scratch state of re-advanced segments and the tail's start state
+    std::vector<void *> adj_slot, adj_range;
+    int64_t adj_range_rb = 0;
+    void *adj_work = nullptr, *adj_tstart = nullptr;
+    std::vector<hipEvent_t> adj_range_ev;   // per range: recomputed, reversed
+    int64_t adj_budget = 0;                 // bytes (0: one checkpoint per segment)
+    smp::AdjSchedule adj_sched;
+    int64_t adj_peak_bytes = 0;
     std::map<const void *, void *> adj_dE;   // dE/dsigma field of each sigma field
     double adj_fwd_ms = 0, adj_rev_ms = 0;
     double last_total_ms = 0, last_substep_ms = 0, last_bytes = 0;
@@ -139,6 +148,7 @@ struct smp_problem {
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
         for (hipEvent_t e : adj_ev)
             if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t e : adj_range_ev) (void)hipEventDestroy(e);
         if (adj_stream) (void)hipStreamDestroy(adj_stream);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -1233,43 +1243,129 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
     rc = adjoint_tables<T>(h, dt);
     if (rc) return rc;
     const size_t st = (size_t)h->slab * h->elem * h->geo.n_sys;
-    const int64_t n_seg = (n_steps + every - 1) / every;
-    // checkpoint buffers: one state per segment start, two per step of a
-    // segment (before substep 1, after substep 2)
-    while ((int64_t)h->adj_seg.size() < n_seg) {
+    smp::AdjSchedule sc;
+    {
+        const int64_t nbuf = h->adj_budget > 0 ? h->adj_budget / (int64_t)st : -1;
+        const std::string err = smp::plan_adjoint(n_steps, every, nbuf, sc);
+        if (!err.empty()) return fail(h->adj_budget > 0 ? SMP_E_OOM : SMP_E_INVALID, err);
+    }
+    const int64_t k = sc.seg_steps, rb = k + 1;
+    const int H = sc.segments - sc.tail;
+    // buffers of the schedule, kept on the handle for the next call
+    while ((int)h->adj_slot.size() < sc.slots) {
         void *p = nullptr;
         rc = h->dalloc(&p, st);
         if (rc) return rc;
-        h->adj_seg.push_back(p);
+        h->adj_slot.push_back(p);
     }
-    // records of two segments' recomputations (the next segment is
-    // recomputed while the current one is stepped back): the targets of
-    // every launch (2*every+1 launches) after it, written there by the
-    // launch itself; the other parity's slabs of a record are never written
-    // and stay zero
-    const int64_t rec_per_set = 2 * (int64_t)every + 1;
-    while ((int64_t)h->adj_rec.size() < 2 * rec_per_set) {
+    if (h->adj_range_rb != rb) {
+        for (void *p : h->adj_range) h->dfree(p);
+        h->adj_range.clear();
+        h->adj_range_rb = rb;
+    }
+    while ((int)h->adj_range.size() < sc.ranges) {
         void *p = nullptr;
-        rc = h->dalloc(&p, st);
+        rc = h->dalloc(&p, st * (size_t)rb);
         if (rc) return rc;
-        HIP_TRY(hipMemsetAsync(p, 0, st, h->stream));
-        h->adj_rec.push_back(p);
+        h->adj_range.push_back(p);
     }
+    if (sc.work && !h->adj_work) {
+        rc = h->dalloc(&h->adj_work, st);
+        if (rc) return rc;
+    }
+    if (sc.tail > 0 && !h->adj_tstart) {
+        rc = h->dalloc(&h->adj_tstart, st);
+        if (rc) return rc;
+    }
+    h->adj_sched = sc;
+    h->adj_peak_bytes = sc.buffers * (int64_t)st;
     for (int q = 0; q < 8; ++q)
         if (!h->adj_ev[q])
             HIP_TRY(q < 3 ? hipEventCreate(&h->adj_ev[q])
                           : hipEventCreateWithFlags(&h->adj_ev[q], hipEventDisableTiming));
-    hipEvent_t e0 = h->adj_ev[0], e1 = h->adj_ev[1], e2 = h->adj_ev[2], ev_fwd = h->adj_ev[3];
-    hipEvent_t ev_rc[2] = {h->adj_ev[4], h->adj_ev[5]}, ev_rev[2] = {h->adj_ev[6], h->adj_ev[7]};
-    HIP_TRY(hipEventRecord(e0, h->stream));
-    // forward pass: segment checkpoints, the state ends at t = n_steps*dt
-    for (int64_t sg = 0; sg < n_seg; ++sg) {
-        HIP_TRY(hipMemcpyAsync(h->adj_seg[sg], h->U, st, hipMemcpyDeviceToDevice, h->stream));
-        const int64_t ns = std::min<int64_t>(every, n_steps - sg * every);
-        rc = run_steps<T>(h, dt, ns);
-        if (rc) return rc;
+    while ((int)h->adj_range_ev.size() < 2 * sc.ranges) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        h->adj_range_ev.push_back(e);
     }
-    HIP_TRY(hipEventRecord(e1, h->stream));
+    hipEvent_t e0 = h->adj_ev[0], e1 = h->adj_ev[1], e2 = h->adj_ev[2], ev_fwd = h->adj_ev[3];
+    if (!h->adj_stream) HIP_TRY(hipStreamCreateWithFlags(&h->adj_stream, hipStreamNonBlocking));
+    hipStream_t rs = h->adj_stream;
+    const smp::SubstepArgs<T> f_odd_first = make_args<T>(h, 0, dt, 1, 0, 0);
+    const smp::SubstepArgs<T> f_even = make_args<T>(h, 1, dt, 2, 0, 1);
+    const smp::SubstepArgs<T> f_odd_fused = make_args<T>(h, 0, dt, 2, 1, 0);
+    const smp::SubstepArgs<T> f_odd_last = make_args<T>(h, 0, dt, 1, 1, 1);
+    // A segment of m steps is 2m+1 launches (odd first; the even substep and
+    // the fused odd substeps of consecutive steps; odd last): launch L updates
+    // the targets of one parity (odd for even L) from the other parity.
+    auto seg_len = [&](int64_t sg) { return std::min<int64_t>(k, n_steps - sg * k); };
+    auto nl_of = [&](int64_t sg) { return 2 * seg_len(sg) + 1; };
+    auto launch_of = [&](int64_t L, int64_t nl) -> const smp::SubstepArgs<T> & {
+        if (L == 0) return f_odd_first;
+        if (L == nl - 1) return f_odd_last;
+        return L % 2 ? f_even : f_odd_fused;
+    };
+    // record L of a range: the targets of launch L after it; records 2b and
+    // 2b+1 (odd and even targets) share buffer b of the range
+    auto rec = [&](int r, int64_t L) -> void * {
+        return (char *)h->adj_range[r] + (size_t)(L / 2) * st;
+    };
+    auto buf_of = [&](int id) -> void * {
+        if (id >= 0) return h->adj_slot[id];
+        if (id == smp::ADJ_WORK) return h->adj_work;
+        if (id == smp::ADJ_TAIL_START) return h->adj_tstart;
+        return nullptr;
+    };
+    // where each parity of a segment's start state lives: a full state, or
+    // the previous (recorded) segment's last two records
+    struct Start {
+        void *odd, *even;
+    };
+    auto start_of = [&](int64_t sg, int src) -> Start {
+        if (src == smp::ADJ_PREV) {
+            const int64_t nlp = nl_of(sg - 1);
+            const int rp = (int)(sg - 1 - H);
+            return Start{rec(rp, nlp - 1), rec(rp, nlp - 2)};
+        }
+        void *b = buf_of(src);
+        return Start{b, b};
+    };
+    // the state after launch m of a segment, for the parity launch m targets
+    // (m = -2: odd, m = -1: even, at the segment start)
+    auto after = [&](const Start &s0, int r, int64_t m) -> void * {
+        if (m >= 0) return rec(r, m);
+        return m == -2 ? s0.odd : s0.even;
+    };
+    // segment sg out of place into range r: launch L reads its targets after
+    // launch L-2 and its sources after launch L-1, and writes record L
+    auto record_seg = [&](int64_t sg, const Start &s0, int r, hipStream_t strm) -> int {
+        const int64_t nl = nl_of(sg);
+        for (int64_t L = 0; L < nl; ++L) {
+            const int kind = L % 2;
+            smp::SubstepArgs<T> a = launch_of(L, nl);
+            a.U = (T *)after(s0, r, L - 2);
+            a.src = (const T *)after(s0, r, L - 1);
+            a.out = (T *)rec(r, L);
+            HIP_TRY(smp::launch_substep<T>(a, h->plan[kind].max_terms, h->gen[kind], h->cell_waves,
+                                           h->lds_bytes[kind], strm));
+        }
+        return SMP_OK;
+    };
+    // segments [s0, s1) in place on the state in buf
+    auto run_inplace = [&](int64_t s0, int64_t s1, void *buf, hipStream_t strm) -> int {
+        for (int64_t sg = s0; sg < s1; ++sg) {
+            const int64_t nl = nl_of(sg);
+            for (int64_t L = 0; L < nl; ++L) {
+                const int kind = L % 2;
+                smp::SubstepArgs<T> a = launch_of(L, nl);
+                a.U = a.out = (T *)buf;
+                a.src = (const T *)buf;
+                HIP_TRY(smp::launch_substep<T>(a, h->plan[kind].max_terms, h->gen[kind], h->cell_waves,
+                                               h->lds_bytes[kind], strm));
+            }
+        }
+        return SMP_OK;
+    };
     const int nl_odd = (int)h->geo.odd.size(), nl_even = (int)h->geo.even.size();
     const int32_t *list_odd = (const int32_t *)h->adj_list;
     const int32_t *list_even = list_odd + nl_odd;
@@ -1307,73 +1403,18 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
         a.ndim = h->geo.ndim;
         return a;
     };
-    // reverse sweep, one segment at a time.  The segment's launches are the
-    // forward's (odd first, then even and fused odd per step, odd last):
-    // launch L updates the targets of one parity from the other parity with
-    // napp applications.  The recomputation writes the targets of launch L
-    // to record L, so L's targets before L are in record L-2 and its sources
-    // in record L-1 (the segment's checkpoint where the index is negative).
-    // The adjoint steps back launch by launch: the update of L's targets,
-    // then the transposed flux into L's sources.
-    // Two record sets: segment sg uses set sg % 2.  The recomputation runs
-    // on its own stream one segment ahead of the reverse sweep, which runs
-    // on the handle's stream.
-    if (!h->adj_stream) HIP_TRY(hipStreamCreateWithFlags(&h->adj_stream, hipStreamNonBlocking));
-    hipStream_t rs = h->adj_stream;
-    HIP_TRY(hipEventRecord(ev_fwd, h->stream));
-    HIP_TRY(hipStreamWaitEvent(rs, ev_fwd, 0));
-    const smp::SubstepArgs<T> f_odd_first = make_args<T>(h, 0, dt, 1, 0, 0);
-    const smp::SubstepArgs<T> f_even = make_args<T>(h, 1, dt, 2, 0, 1);
-    const smp::SubstepArgs<T> f_odd_fused = make_args<T>(h, 0, dt, 2, 1, 0);
-    const smp::SubstepArgs<T> f_odd_last = make_args<T>(h, 0, dt, 1, 1, 1);
-    auto seg_launches = [&](int64_t sg) { return 2 * std::min<int64_t>(every, n_steps - sg * every) + 1; };
-    auto launch_of = [&](int64_t L, int64_t nl) -> const smp::SubstepArgs<T> & {
-        if (L == 0) return f_odd_first;
-        if (L == nl - 1) return f_odd_last;
-        return L % 2 ? f_even : f_odd_fused;
-    };
-    auto rec = [&](int q, int64_t L) { return h->adj_rec[q * rec_per_set + L]; };
-    // the state of segment sg after its launch m (m < 0: the segment's
-    // checkpoint), for the parity launch m targets
-    auto after = [&](int64_t sg, int64_t m) { return m >= 0 ? rec((int)(sg % 2), m) : h->adj_seg[sg]; };
-    // recompute segment sg on the recomputation stream, out of place: launch
-    // L reads its targets after launch L-2 and its sources after launch L-1
-    // and writes its targets to record L
-    auto recompute = [&](int64_t sg) -> int {
-        const int q = (int)(sg % 2);
-        const int64_t nl = seg_launches(sg);
-        for (int64_t L = 0; L < nl; ++L) {
-            const int kind = L % 2;   // 0: odd targets
-            smp::SubstepArgs<T> a = launch_of(L, nl);
-            a.U = (T *)after(sg, L - 2);
-            a.src = (const T *)after(sg, L - 1);
-            a.out = (T *)rec(q, L);
-            HIP_TRY(smp::launch_substep<T>(a, h->plan[kind].max_terms, h->gen[kind], h->cell_waves,
-                                           h->lds_bytes[kind], rs));
-        }
-        HIP_TRY(hipEventRecord(ev_rc[q], rs));
-        return SMP_OK;
-    };
-    rc = recompute(n_seg - 1);
-    if (rc) return rc;
-    for (int64_t sg = n_seg - 1; sg >= 0; --sg) {
-        const int q = (int)(sg % 2);
-        const int64_t nl = seg_launches(sg);
-        // the next segment's recomputation reuses the set of segment sg+1
-        if (sg >= 1) {
-            if (sg + 1 < n_seg) HIP_TRY(hipStreamWaitEvent(rs, ev_rev[(sg + 1) % 2], 0));
-            rc = recompute(sg - 1);
-            if (rc) return rc;
-        }
-        HIP_TRY(hipStreamWaitEvent(h->stream, ev_rc[q], 0));
-        // the transposed flux of launch L+1 is fused into the update of L
-        // (both touch lambda of L's targets); the segment's first launch
-        // ends with it as a launch of its own
+    // the adjoint of segment sg, launch by launch from the last: the update
+    // of L's targets (L's targets before L and its sources are the states
+    // after launches L-2 and L-1), with the transposed flux of launch L+1
+    // fused in; the segment's first launch ends with the transposed flux
+    // into its sources as a launch of its own
+    auto reverse_seg = [&](int64_t sg, const Start &s0, int r) -> int {
+        const int64_t nl = nl_of(sg);
         for (int64_t L = nl - 1; L >= 0; --L) {
             const int kind = L % 2;
             const smp::SubstepArgs<T> &f = launch_of(L, nl);
-            const void *U0 = after(sg, L - 2);   // L's targets before L
-            const void *U = after(sg, L - 1);    // L's sources
+            const void *U0 = after(s0, r, L - 2);
+            const void *U = after(s0, r, L - 1);
             const int32_t *tl = kind ? list_even : list_odd, *sl = kind ? list_odd : list_even;
             const int ntl = kind ? nl_even : nl_odd, nsl = kind ? nl_odd : nl_even;
             HIP_TRY(smp::launch_adj_update<T>(adj_args(U, U0, tl, ntl, f.napp, f.fslot[0], f.fslot[1]),
@@ -1383,8 +1424,82 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
                                                               f.fslot[1]),
                                                      h->stream));
         }
-        HIP_TRY(hipEventRecord(ev_rev[q], h->stream));
+        return SMP_OK;
+    };
+
+    // forward pass (the handle's stream): checkpoints and in-place segments
+    // of the head, the tail recorded out of place, the final state
+    HIP_TRY(hipEventRecord(e0, h->stream));
+    for (const smp::AdjAction &a : sc.fwd) {
+        switch (a.kind) {
+        case smp::ADJ_FWD_RUN:
+            rc = run_inplace(a.s0, a.s1, h->U, h->stream);
+            break;
+        case smp::ADJ_FWD_CKPT:
+            HIP_TRY(hipMemcpyAsync(buf_of(a.dst), h->U, st, hipMemcpyDeviceToDevice, h->stream));
+            break;
+        case smp::ADJ_FWD_REC:
+            rc = record_seg(a.s0, start_of(a.s0, a.src), a.range, h->stream);
+            break;
+        case smp::ADJ_FWD_FINAL: {
+            const int64_t nl = nl_of(a.s0);
+            const size_t sb = (size_t)h->slab * h->elem;
+            for (int par = 0; par < 2; ++par)
+                for (int j : par == 0 ? h->geo.odd : h->geo.even)
+                    HIP_TRY(hipMemcpyAsync((char *)h->U + j * sb,
+                                           (const char *)rec(a.range, par == 0 ? nl - 1 : nl - 2) + j * sb,
+                                           sb, hipMemcpyDeviceToDevice, h->stream));
+            break;
+        }
+        default:
+            return fail(SMP_E_INVALID, "adjoint schedule: unexpected forward action");
+        }
+        if (rc) return rc;
     }
+    HIP_TRY(hipEventRecord(e1, h->stream));
+    HIP_TRY(hipEventRecord(ev_fwd, h->stream));
+    HIP_TRY(hipStreamWaitEvent(rs, ev_fwd, 0));
+    // reverse sweep (the handle's stream) with the recomputation and the
+    // re-advanced segments on the second stream: a recomputation into a
+    // range waits until the range's previous segment is stepped back, a
+    // reversal until its range is recomputed, and a re-advance until every
+    // reversal issued before it is done (it may overwrite a start state
+    // that a reversal still reads)
+    std::vector<char> rc_pending(sc.ranges, 0), rev_done(sc.ranges, 0);
+    hipEvent_t last_rev = nullptr;
+    for (const smp::AdjAction &a : sc.rev) {
+        switch (a.kind) {
+        case smp::ADJ_ADVANCE:
+            if (last_rev) HIP_TRY(hipStreamWaitEvent(rs, last_rev, 0));
+            HIP_TRY(hipMemcpyAsync(buf_of(a.dst), buf_of(a.src), st, hipMemcpyDeviceToDevice, rs));
+            rc = run_inplace(a.s0, a.s1, buf_of(a.dst), rs);
+            break;
+        case smp::ADJ_RECOMP: {
+            if (rev_done[a.range]) HIP_TRY(hipStreamWaitEvent(rs, h->adj_range_ev[2 * a.range + 1], 0));
+            rc = record_seg(a.s0, start_of(a.s0, a.src), a.range, rs);
+            if (rc) break;
+            HIP_TRY(hipEventRecord(h->adj_range_ev[2 * a.range], rs));
+            rc_pending[a.range] = 1;
+            break;
+        }
+        case smp::ADJ_REVERSE: {
+            if (rc_pending[a.range]) {
+                HIP_TRY(hipStreamWaitEvent(h->stream, h->adj_range_ev[2 * a.range], 0));
+                rc_pending[a.range] = 0;
+            }
+            rc = reverse_seg(a.s0, start_of(a.s0, a.src), a.range);
+            if (rc) break;
+            HIP_TRY(hipEventRecord(h->adj_range_ev[2 * a.range + 1], h->stream));
+            rev_done[a.range] = 1;
+            last_rev = h->adj_range_ev[2 * a.range + 1];
+            break;
+        }
+        default:
+            return fail(SMP_E_INVALID, "adjoint schedule: unexpected reverse action");
+        }
+        if (rc) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(rs));
     HIP_TRY(hipEventRecord(e2, h->stream));
     HIP_TRY(hipEventSynchronize(e2));
     float f = 0, r = 0;
@@ -2146,6 +2261,52 @@ int smp_adjoint_timing(const smp_problem *h, double *forward_ms, double *reverse
     if (!h) return fail(SMP_E_INVALID, "null handle");
     if (forward_ms) *forward_ms = h->adj_fwd_ms;
     if (reverse_ms) *reverse_ms = h->adj_rev_ms;
+    return SMP_OK;
+}
+
+int smp_adjoint_set_memory(smp_problem *h, int64_t bytes) {
+    if (!h) return fail(SMP_E_INVALID, "null handle");
+    if (bytes < 0) return fail(SMP_E_INVALID, "bytes must be >= 0");
+    h->adj_budget = bytes;
+    return SMP_OK;
+}
+
+static void adjoint_stats_of(const smp::AdjSchedule &sc, int64_t bytes, int64_t stats[8]) {
+    stats[0] = sc.segments;
+    stats[1] = sc.seg_steps;
+    stats[2] = sc.tail;
+    stats[3] = sc.slots;
+    stats[4] = sc.recomputed;
+    stats[5] = sc.extra_advances;
+    stats[6] = sc.buffers;
+    stats[7] = bytes;
+}
+
+int smp_adjoint_stats(const smp_problem *h, int64_t stats[8]) {
+    if (!h || !stats) return fail(SMP_E_INVALID, "null argument");
+    adjoint_stats_of(h->adj_sched, h->adj_peak_bytes, stats);
+    return SMP_OK;
+}
+
+int smp_adjoint_schedule(int64_t n_steps, int32_t every, int64_t buffers, int64_t stats[8],
+                         int32_t *actions, int64_t cap, int64_t *n_fwd, int64_t *n_actions) {
+    if (!stats || !n_fwd || !n_actions) return fail(SMP_E_INVALID, "null argument");
+    smp::AdjSchedule sc;
+    const std::string err = smp::plan_adjoint(n_steps, every, buffers, sc);
+    if (!err.empty()) return fail(buffers >= 0 ? SMP_E_OOM : SMP_E_INVALID, err);
+    adjoint_stats_of(sc, 0, stats);
+    *n_fwd = (int64_t)sc.fwd.size();
+    *n_actions = (int64_t)(sc.fwd.size() + sc.rev.size());
+    if (actions) {
+        if (cap < *n_actions) return fail(SMP_E_INVALID, "actions buffer too small");
+        int64_t q = 0;
+        for (const auto *v : {&sc.fwd, &sc.rev})
+            for (const smp::AdjAction &a : *v) {
+                const int32_t row[6] = {a.kind, a.s0, a.s1, a.src, a.dst, a.range};
+                std::memcpy(actions + 6 * q, row, sizeof(row));
+                ++q;
+            }
+    }
     return SMP_OK;
 }
 

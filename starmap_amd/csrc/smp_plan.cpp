@@ -2,6 +2,7 @@
 #include "smp_plan.h"
 
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <deque>
 #include <limits>
@@ -275,6 +276,223 @@ std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slo
         out.groups.push_back(gr);
     }
     out.n_slots = used;
+    return "";
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint schedule (see smp_plan.h)
+
+namespace {
+
+// Extra segment advances of reversing l segments from a checkpoint at their
+// start with f further checkpoint slots: N (the segments were not passed by
+// the forward pass in order) and S (they lie on the forward pass's way, so
+// advancing through them from the left is free).  A split at m stores the
+// state after m segments in a slot, reverses the right part with one slot
+// less, frees the slot and reverses the left part.  Without a free slot
+// every segment is re-advanced from the checkpoint.
+struct AdjCost {
+    int L = 0, F = 0;
+    std::vector<int64_t> N, S;
+    std::vector<int32_t> mN, mS;
+    int64_t &n(int l, int f) { return N[(size_t)l * (F + 1) + f]; }
+    int64_t &s(int l, int f) { return S[(size_t)l * (F + 1) + f]; }
+    int32_t &sn(int l, int f) { return mN[(size_t)l * (F + 1) + f]; }
+    int32_t &ss(int l, int f) { return mS[(size_t)l * (F + 1) + f]; }
+    void build(int l_max, int f_max) {
+        L = l_max;
+        F = f_max;
+        const size_t sz = (size_t)(L + 1) * (F + 1);
+        N.assign(sz, 0);
+        S.assign(sz, 0);
+        mN.assign(sz, 0);
+        mS.assign(sz, 0);
+        for (int l = 1; l <= L; ++l)
+            for (int f = 0; f <= F; ++f) {
+                if (l == 1) continue;
+                if (f == 0) {
+                    n(l, f) = s(l, f) = (int64_t)l * (l - 1) / 2;
+                    continue;
+                }
+                if (f >= l - 1) {
+                    // a slot at every segment start on the way
+                    n(l, f) = l - 1;
+                    s(l, f) = 0;
+                    sn(l, f) = ss(l, f) = 1;
+                    continue;
+                }
+                int64_t bn = INT64_MAX, bs = INT64_MAX;
+                for (int m = 1; m < l; ++m) {
+                    const int64_t cn = m + n(l - m, f - 1) + n(m, f);
+                    const int64_t cs = s(l - m, f - 1) + n(m, f);
+                    if (cn < bn) { bn = cn; sn(l, f) = m; }
+                    if (cs < bs) { bs = cs; ss(l, f) = m; }
+                }
+                n(l, f) = bn;
+                s(l, f) = bs;
+            }
+    }
+};
+
+struct AdjEmitter {
+    AdjSchedule &out;
+    AdjCost &cost;
+    std::vector<int> free_slots;
+    int head_reversed = 0;
+    int hr[2] = {0, 1};
+    std::vector<AdjAction> head_rev;
+
+    AdjEmitter(AdjSchedule &o, AdjCost &c) : out(o), cost(c) {}
+    static AdjAction act(int kind, int s0, int s1, int src, int dst, int range) {
+        AdjAction a;
+        a.kind = kind;
+        a.s0 = s0;
+        a.s1 = s1;
+        a.src = src;
+        a.dst = dst;
+        a.range = range;
+        return a;
+    }
+    int take_slot() {
+        const int s = free_slots.back();
+        free_slots.pop_back();
+        return s;
+    }
+    void reverse_one(int seg, int src) {
+        const int r = hr[head_reversed++ % 2];
+        head_rev.push_back(act(ADJ_RECOMP, seg, seg + 1, src, ADJ_NONE, r));
+        head_rev.push_back(act(ADJ_REVERSE, seg, seg + 1, src, ADJ_NONE, r));
+    }
+    // reverse segments [a, b) from the checkpoint of a in slot sa, with f
+    // free slots; spine: the forward pass still has to pass through them
+    void rh(int a, int b, int sa, int f, bool spine) {
+        const int l = b - a;
+        if (l == 1) {
+            if (spine) out.fwd.push_back(act(ADJ_FWD_RUN, a, b, ADJ_NONE, ADJ_NONE, ADJ_NONE));
+            reverse_one(a, sa);
+            return;
+        }
+        if (f == 0) {
+            if (spine) out.fwd.push_back(act(ADJ_FWD_RUN, a, b, ADJ_NONE, ADJ_NONE, ADJ_NONE));
+            for (int sg = b - 1; sg >= a; --sg) {
+                int src = sa;
+                if (sg > a) {
+                    head_rev.push_back(act(ADJ_ADVANCE, a, sg, sa, ADJ_WORK, ADJ_NONE));
+                    out.extra_advances += sg - a;
+                    out.work = 1;
+                    src = ADJ_WORK;
+                }
+                reverse_one(sg, src);
+            }
+            return;
+        }
+        // with a slot for every segment start, store each one on the way
+        const int m = f >= l - 1 ? a + 1 : a + (spine ? cost.ss(l, f) : cost.sn(l, f));
+        const int sm = take_slot();
+        if (spine) {
+            out.fwd.push_back(act(ADJ_FWD_RUN, a, m, ADJ_NONE, ADJ_NONE, ADJ_NONE));
+            out.fwd.push_back(act(ADJ_FWD_CKPT, m, m, ADJ_NONE, sm, ADJ_NONE));
+        } else {
+            head_rev.push_back(act(ADJ_ADVANCE, a, m, sa, sm, ADJ_NONE));
+            out.extra_advances += m - a;
+        }
+        rh(m, b, sm, f - 1, spine);
+        free_slots.push_back(sm);
+        rh(a, m, sa, f, false);
+    }
+};
+
+}  // namespace
+
+std::string plan_adjoint(int64_t n_steps, int every, int64_t buffers, AdjSchedule &out) {
+    out = AdjSchedule();
+    if (n_steps < 1) return "n_steps must be >= 1";
+    if (every < 1) return "checkpoint_every must be >= 1";
+    const int64_t k = std::min<int64_t>(every, n_steps);
+    const int64_t U64 = (n_steps + k - 1) / k;
+    if (U64 > 1 << 20) return "too many segments: raise checkpoint_every";
+    const int U = (int)U64;
+    const int64_t rb = k + 1;   // buffers of one record range
+    out.segments = U;
+    out.seg_steps = (int32_t)k;
+    auto ranges_of = [&](int T, int H) -> int {
+        if (H == 0) return T;
+        return T >= 1 ? std::max(T, 3) : 2;
+    };
+    int best_T = -1, best_c = 0;
+    int64_t best_cost = INT64_MAX;
+    AdjCost cost;
+    if (buffers < 0) {
+        best_T = 0;
+        best_c = U;
+        best_cost = U;
+    } else {
+        // the dynamic programme covers head ranges of up to 4096 segments
+        for (int T = U; T >= 0; --T) {
+            const int H = U - T;
+            if (H > 4096) break;
+            const int64_t fixed = ranges_of(T, H) * rb + (T > 0 ? 1 : 0);
+            if (H == 0) {
+                if (fixed <= buffers) {
+                    best_T = T;
+                    best_c = 0;
+                    best_cost = 0;
+                    break;
+                }
+                continue;
+            }
+            int64_t c = std::min<int64_t>(H, buffers - fixed);
+            if (c < H) c = std::min<int64_t>(H, buffers - fixed - 1);   // and the scratch state
+            if (c < 1) continue;
+            int64_t extra = 0;
+            if (c < H) {
+                if (cost.L < H || cost.F < c - 1) cost.build(H, (int)std::min<int64_t>(c - 1, H - 1));
+                extra = cost.s(H, (int)std::min<int64_t>(c - 1, cost.F));
+            }
+            const int64_t tot = H + extra;
+            if (tot < best_cost) {
+                best_cost = tot;
+                best_T = T;
+                best_c = (int)c;
+            }
+        }
+        if (best_T < 0) {
+            std::ostringstream os;
+            os << "adjoint memory budget too small: " << buffers << " state buffers, at least "
+               << (std::min<int64_t>(U, 1) + 2 * rb + 2) << " needed for segments of " << k << " steps";
+            return os.str();
+        }
+    }
+    const int T = best_T, H = U - T, c = best_c;
+    out.tail = T;
+    out.slots = c;
+    out.recomputed = H;
+    out.ranges = ranges_of(T, H);
+    if (H > 0 && c < H && (cost.L < H || cost.F < c - 1))
+        cost.build(H, std::min(c - 1, H - 1));
+    AdjEmitter em(out, cost);
+    if (T >= 1) {
+        em.hr[0] = 1;
+        em.hr[1] = 2;
+    }
+    if (H > 0) {
+        for (int s = c - 1; s >= 0; --s) em.free_slots.push_back(s);
+        const int s0 = em.take_slot();
+        out.fwd.push_back(AdjEmitter::act(ADJ_FWD_CKPT, 0, 0, ADJ_NONE, s0, ADJ_NONE));
+        em.rh(0, H, s0, c - 1, true);
+    }
+    if (T > 0) {
+        out.fwd.push_back(AdjEmitter::act(ADJ_FWD_CKPT, H, H, ADJ_NONE, ADJ_TAIL_START, ADJ_NONE));
+        for (int s = H; s < U; ++s)
+            out.fwd.push_back(AdjEmitter::act(ADJ_FWD_REC, s, s + 1, s == H ? ADJ_TAIL_START : ADJ_PREV,
+                                              ADJ_NONE, s - H));
+        out.fwd.push_back(AdjEmitter::act(ADJ_FWD_FINAL, U - 1, U, ADJ_NONE, ADJ_NONE, T - 1));
+        for (int s = U - 1; s >= H; --s)
+            out.rev.push_back(AdjEmitter::act(ADJ_REVERSE, s, s + 1, s == H ? ADJ_TAIL_START : ADJ_PREV,
+                                              ADJ_NONE, s - H));
+    }
+    out.rev.insert(out.rev.end(), em.head_rev.begin(), em.head_rev.end());
+    out.buffers = out.ranges * rb + (T > 0 ? 1 : 0) + c + out.work;
     return "";
 }
 

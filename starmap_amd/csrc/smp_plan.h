@@ -119,4 +119,54 @@ struct Plan {
 std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slot_capacity,
                        int group_size, int max_load_elems, int max_loads_per_group, Plan &out);
 
+// ---- discrete adjoint: checkpoints, recorded segments, recomputation -----
+// The adjoint steps back over segments of `every` steps.  Reversing a
+// segment needs the forward's launch outputs ("records") of that segment;
+// a segment of m steps has 2m+1 launches, and two consecutive records (one
+// per parity) share one state-sized buffer, so its records take m+1
+// buffers.  Under a budget of B state-sized buffers the schedule
+//   * records the last T segments ("tail") during the forward pass itself:
+//     they are never recomputed;
+//   * keeps c checkpoint slots for the first H = U - T segments ("head"),
+//     reversed from last to first, each recomputed once into one of two
+//     alternating record ranges; with fewer slots than head segments the
+//     slots are placed by binomial checkpointing (the optimal split of each
+//     range of segments, found by dynamic programming over the extra segment
+//     advances), which costs further forward advances from a checkpoint.
+// T and c minimise the recomputed and re-advanced segments within B.
+enum AdjActKind {
+    ADJ_FWD_RUN = 0,    // forward pass: segments [s0, s1) in place on the state
+    ADJ_FWD_CKPT = 1,   // forward pass: copy the state into buffer dst
+    ADJ_FWD_REC = 2,    // forward pass: segment s0 out of place into record range `range`
+    ADJ_FWD_FINAL = 3,  // forward pass: the final state from record range `range` into the state
+    ADJ_ADVANCE = 4,    // reverse: copy buffer src into dst, then segments [s0, s1) in place on dst
+    ADJ_RECOMP = 5,     // reverse: segment s0 from buffer src into record range `range`
+    ADJ_REVERSE = 6,    // reverse: adjoint of segment s0 from record range `range`, start state src
+};
+// buffers other than checkpoint slots (>= 0)
+constexpr int ADJ_NONE = -1;
+constexpr int ADJ_WORK = -2;        // scratch state of the re-advanced segments
+constexpr int ADJ_TAIL_START = -3;  // the state at the start of the tail
+constexpr int ADJ_PREV = -4;        // the previous segment's last two records
+
+struct AdjAction {
+    int32_t kind, s0, s1, src, dst, range;
+};
+
+struct AdjSchedule {
+    int32_t segments = 0;       // U
+    int32_t seg_steps = 0;      // steps per segment (the last may be shorter)
+    int32_t tail = 0;           // T
+    int32_t slots = 0;          // c
+    int32_t work = 0;           // 1 if the schedule uses the scratch state
+    int32_t ranges = 0;         // record ranges of seg_steps+1 buffers each
+    int64_t recomputed = 0;     // head segments recomputed into records
+    int64_t extra_advances = 0; // further segment advances from checkpoints
+    int64_t buffers = 0;        // state-sized buffers at the peak
+    std::vector<AdjAction> fwd, rev;
+};
+
+// buffers < 0: no budget (a checkpoint per segment, no tail)
+std::string plan_adjoint(int64_t n_steps, int every, int64_t buffers, AdjSchedule &out);
+
 }  // namespace smp

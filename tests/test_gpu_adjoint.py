@@ -303,3 +303,42 @@ def test_torch_loss_of_final_moment_drives_the_adjoint():
             e = 1e-4 * base.flat[c]
             fd = (4 * central(e) - central(2 * e)) / 3
             assert abs(float(grad.numpy().flat[c]) - fd) <= 1e-7 * abs(fd) + 1e-10, (which, c)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_schedules_under_memory_budgets_give_identical_results(precision):
+    """The same adjoint under four schedules -- a checkpoint per segment (no
+    budget), everything recorded in the forward pass, a few segments recorded
+    and the rest recomputed, and binomial checkpointing with re-advanced
+    segments -- gives the same lambda(0), gradients and final state bit for
+    bit: every schedule recomputes each launch from the same inputs."""
+    prob = P.adjoint_grf(n=32, n_mom=5, steps=37, src_std=0.2)
+    every = 4                              # 10 segments, the last of 1 step
+    res = {}
+    s = Adjoint(I.starmap_init(prob), precision=precision)
+    try:
+        mask = strip_objective(s, xmin=0.2)
+        s.run({0: mask}, every=every)
+        st0 = s.schedule_stats()
+        state_bytes = st0["peak_bytes"] // st0["peak_buffers"]
+        assert st0["recomputed_segments"] == 10 and st0["recorded_in_forward"] == 0
+        budgets = {"default": None, "all": 10 * 5 + 1, "some": 6 * 5 + 1 + 4, "binomial": 2 * 5 + 4}
+        for name, nbuf in budgets.items():
+            s.set_memory(0 if nbuf is None else nbuf * state_bytes)
+            s.run({0: mask}, every=every)
+            st = s.schedule_stats()
+            print(name, st, flush=True)
+            if nbuf is not None:
+                assert st["peak_bytes"] <= nbuf * state_bytes
+            res[name] = ([s.lam(j) for j in range(s.n_sys)],
+                         [g for pair in s.material_gradients().values() for g in pair],
+                         [s.get_state(j) for j in range(s.n_sys)], st)
+    finally:
+        s.close()
+    assert res["all"][3]["recomputed_segments"] == 0
+    assert 0 < res["some"][3]["recorded_in_forward"] < 10
+    assert res["binomial"][3]["extra_segment_advances"] > 0
+    ref = res["default"]
+    for name in ("all", "some", "binomial"):
+        for a, b in zip(ref[0] + ref[1] + ref[2], res[name][0] + res[name][1] + res[name][2]):
+            assert np.array_equal(a, b), name
