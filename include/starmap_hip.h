@@ -283,7 +283,10 @@ int smp_adjoint_stats(const smp_problem *h, int64_t stats[8]);
 
 /* Host-only planner behind smp_adjoint: the schedule of n_steps steps in
  * segments of `every` steps within `buffers` state-sized buffers (-1: no
- * budget).  stats as smp_adjoint_stats (bytes 0).  If actions is not null it
+ * budget), a step's records taking rec_factor buffers (1: the launches'
+ * states; 2: and the W+Q the launches stored, which smp_adjoint records
+ * unless coefficients are read from global memory).  stats as
+ * smp_adjoint_stats (bytes 0).  If actions is not null it
  * receives 6 int32 per action {kind, first segment, end segment, source
  * buffer, destination buffer, record range}: the forward pass (*n_fwd
  * actions), then the reverse sweep (*n_actions in all).  Kinds: 0 run in
@@ -291,8 +294,9 @@ int smp_adjoint_stats(const smp_problem *h, int64_t stats[8]);
  * records, 4 copy and re-advance, 5 recompute into records, 6 reverse.
  * Buffers: >= 0 checkpoint slots, -2 scratch state, -3 the state at the
  * start of the recorded segments, -4 the previous segment's records. */
-int smp_adjoint_schedule(int64_t n_steps, int32_t every, int64_t buffers, int64_t stats[8],
-                         int32_t *actions, int64_t cap, int64_t *n_fwd, int64_t *n_actions);
+int smp_adjoint_schedule(int64_t n_steps, int32_t every, int64_t buffers, int32_t rec_factor,
+                         int64_t stats[8], int32_t *actions, int64_t cap, int64_t *n_fwd,
+                         int64_t *n_actions);
 
 /* Moment norms of the current state, per component j in the reference's
  * output layout: sumsq[j] = sum of U_j^2 over its grid points, diffsq[3*j+e]

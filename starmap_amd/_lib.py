@@ -110,7 +110,7 @@ def lib():
         "smp_num_components": (ctypes.c_int, [vp, vp]),
         "smp_adjoint_set_memory": (ctypes.c_int, [vp, i64]),
         "smp_adjoint_stats": (ctypes.c_int, [vp, vp]),
-        "smp_adjoint_schedule": (ctypes.c_int, [i64, i32, i64, vp, vp, i64, vp, vp]),
+        "smp_adjoint_schedule": (ctypes.c_int, [i64, i32, i64, i32, vp, vp, i64, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -27,19 +27,20 @@ from .solver import HipSolver
 ACTION_KINDS = ("run", "checkpoint", "record", "final", "advance", "recompute", "reverse")
 
 
-def adjoint_schedule(n_steps, every, buffers):
+def adjoint_schedule(n_steps, every, buffers, rec_factor=1):
     """The host planner behind smp_adjoint: (stats dict, forward actions,
     reverse actions), each action (kind, first segment, end segment, source
-    buffer, destination buffer, record range); buffers -1: no budget."""
+    buffer, destination buffer, record range); buffers -1: no budget;
+    rec_factor 2: a step's records hold the launches' W+Q too."""
     L = _lib.lib()
     st = np.zeros(8, dtype=np.int64)
     nf, na = ctypes.c_int64(), ctypes.c_int64()
-    _lib.check(L.smp_adjoint_schedule(int(n_steps), int(every), int(buffers), st.ctypes.data,
-                                      None, 0, ctypes.byref(nf), ctypes.byref(na)))
+    _lib.check(L.smp_adjoint_schedule(int(n_steps), int(every), int(buffers), int(rec_factor),
+                                      st.ctypes.data, None, 0, ctypes.byref(nf), ctypes.byref(na)))
     acts = np.zeros((max(1, na.value), 6), dtype=np.int32)
-    _lib.check(L.smp_adjoint_schedule(int(n_steps), int(every), int(buffers), st.ctypes.data,
-                                      acts.ctypes.data, acts.shape[0], ctypes.byref(nf),
-                                      ctypes.byref(na)))
+    _lib.check(L.smp_adjoint_schedule(int(n_steps), int(every), int(buffers), int(rec_factor),
+                                      st.ctypes.data, acts.ctypes.data, acts.shape[0],
+                                      ctypes.byref(nf), ctypes.byref(na)))
     keys = ("segments", "steps_per_segment", "recorded_in_forward", "checkpoint_slots",
             "recomputed_segments", "extra_segment_advances", "peak_buffers", "peak_bytes")
     rows = [(ACTION_KINDS[a[0]],) + tuple(int(v) for v in a[1:]) for a in acts[:na.value]]

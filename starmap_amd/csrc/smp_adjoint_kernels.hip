@@ -45,11 +45,31 @@ __device__ __forceinline__ int adj_index(int i, int d, int n, int bc, int ext) {
     return j < 0 ? 0 : (j >= ext ? ext - 1 : j);
 }
 
-// element at byte offset `off` (< 4 GiB) of the slab of component k
+// The slab of component k as a buffer resource (base in scalar registers,
+// num_records the slab's bytes): a load takes a 32-bit byte offset per lane
+// and no 64-bit address arithmetic, and an offset past the slab (ADJ_OOB)
+// reads zero.  Slabs are below 2 GiB (checked by the host).
+constexpr uint32_t ADJ_OOB = 0xFFFFFF00u;
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t slab_rsrc(const T *base, int64_t slab_bytes, int k) {
+    const char *b = reinterpret_cast<const char *>(base) + k * slab_bytes;
+    // raw 32-bit loads: data format 32 in the descriptor's last dword
+    return __builtin_amdgcn_make_buffer_rsrc((void *)b, (short)0, (int)slab_bytes, 0x00020000);
+}
+
+template <typename T>
+__device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+    }
+}
+
+// element at byte offset `off` of the slab of component k
 template <typename T>
 __device__ __forceinline__ T slab_at(const T *base, int64_t slab_bytes, int k, uint32_t off) {
-    const SMP_GLOBAL char *b = glb(reinterpret_cast<const char *>(base)) + k * slab_bytes;
-    return *reinterpret_cast<const SMP_GLOBAL T *>(b + off);
+    return buf_ld<T>(slab_rsrc(base, slab_bytes, k), off);
 }
 
 // Offsets (elements) of the high and low samples of the forward flux of a
@@ -100,32 +120,37 @@ struct Group {
     double coef[ADJ_GROUP];
 };
 
+// the samples of a group of terms; a sample that does not count (mh / ml
+// false, transposed terms) reads zero through an offset past the slab
 template <typename T>
 __device__ __forceinline__ void gather(Group<T> &gr, const T *base, int64_t sb,
-                                       const SMP_GLOBAL AdjTerm *g, int pe, int oh, int ol) {
+                                       const SMP_GLOBAL AdjTerm *g, int pe, int oh, int ol, bool mh,
+                                       bool ml) {
     int k[ADJ_GROUP];
 #pragma unroll
     for (int t = 0; t < ADJ_GROUP; ++t) {
         k[t] = g[t].k;
         gr.coef[t] = g[t].coef;
     }
-    const uint32_t bh = (uint32_t)(pe + oh) * sizeof(T), bl = (uint32_t)(pe + ol) * sizeof(T);
+    const uint32_t bh = mh ? (uint32_t)(pe + oh) * (uint32_t)sizeof(T) : ADJ_OOB;
+    const uint32_t bl = ml ? (uint32_t)(pe + ol) * (uint32_t)sizeof(T) : ADJ_OOB;
 #pragma unroll
     for (int t = 0; t < ADJ_GROUP; ++t) {
-        gr.vh[t] = slab_at(base, sb, k[t], bh);
-        gr.vl[t] = slab_at(base, sb, k[t], bl);
+        const __amdgpu_buffer_rsrc_t r = slab_rsrc(base, sb, k[t]);
+        gr.vh[t] = buf_ld<T>(r, bh);
+        gr.vl[t] = buf_ld<T>(r, bl);
     }
 }
 
 template <typename T>
-__device__ __forceinline__ double reduce(const Group<T> &gr, bool mh, bool ml) {
+__device__ __forceinline__ double reduce(const Group<T> &gr) {
+    // the difference in the state precision (a sample that does not count
+    // was read as zero), the sum in FP64
     double w = 0;
 #pragma unroll
     for (int t = 0; t < ADJ_GROUP; ++t) {
-        double d = 0;
-        if (mh) d += (double)gr.vh[t];
-        if (ml) d -= (double)gr.vl[t];
-        w += gr.coef[t] * d;
+        const T d = gr.vh[t] - gr.vl[t];
+        w += gr.coef[t] * (double)d;
     }
     return w;
 }
@@ -138,7 +163,7 @@ __device__ __forceinline__ void gather_first(Group<T> *g0, const T *base, int64_
                                              const SMP_GLOBAL AdjTerm *terms, const int32_t *ra, int pe,
                                              const AxisOffsets &o) {
 #pragma unroll
-    for (int e = 0; e < NDIM; ++e) gather(g0[e], base, sb, terms + ra[e], pe, o.h[e], o.l[e]);
+    for (int e = 0; e < NDIM; ++e) gather(g0[e], base, sb, terms + ra[e], pe, o.h[e], o.l[e], o.mh[e], o.ml[e]);
 }
 
 template <typename T, int NDIM>
@@ -148,11 +173,11 @@ __device__ __forceinline__ double flux_rest(const Group<T> *g0, const T *base, i
     double w = 0;
 #pragma unroll
     for (int e = 0; e < NDIM; ++e) {
-        w += reduce(g0[e], o.mh[e], o.ml[e]);
+        w += reduce(g0[e]);
         for (int m = ra[e] + ADJ_GROUP; m < ra[e + 1]; m += ADJ_GROUP) {
             Group<T> gr;
-            gather(gr, base, sb, terms + m, pe, o.h[e], o.l[e]);
-            w += reduce(gr, o.mh[e], o.ml[e]);
+            gather(gr, base, sb, terms + m, pe, o.h[e], o.l[e], o.mh[e], o.ml[e]);
+            w += reduce(gr);
         }
     }
     return w;
@@ -164,7 +189,9 @@ __device__ __forceinline__ double flux_rest(const Group<T> *g0, const T *base, i
 // FUSE: first lambda += the transposed flux of the launch after it (its mu,
 // the other parity, terms by source column), as adj_transpose_kernel stores
 // it.
-template <typename T, bool FUSE, int NDIM>
+// HASW: W+Q of every target comes from the forward launch's record (a.W);
+// otherwise the kernel gathers the forward flux itself.
+template <typename T, bool FUSE, int NDIM, bool HASW>
 __global__ __launch_bounds__(256) void adj_update_kernel(const AdjArgs<T> a) {
     // Work order: grid-type groups of the same points next to each other
     // (they share source components), then 64-point chunks along i0, then
@@ -218,23 +245,24 @@ __global__ __launch_bounds__(256) void adj_update_kernel(const AdjArgs<T> a) {
         const bool has_sig = c.sig != nullptr, has_E = c.E != nullptr, has_Q = c.Q != nullptr;
         const T u0_t = slab_at(a.U0, sb, j, pb);
         const T lam_t = slab_at(a.lam, sb, j, pb);
-        const T sig_t = slab_at(has_sig ? c.sig : a.U0, 0, 0, pb);
+        const T sig_t = slab_at(has_sig ? c.sig : a.U0, sb, 0, pb);
         const bool has_dE = c.dE != nullptr;
-        const T E_t = slab_at(has_E ? c.E : a.U0, 0, 0, pb);
-        const T dE_t = slab_at(has_dE ? c.dE : a.U0, 0, 0, pb);
-        const T Q_t = slab_at(has_Q ? c.Q : a.U0, 0, 0, pb);
+        const T E_t = slab_at(has_E ? c.E : a.U0, sb, 0, pb);
+        const T dE_t = slab_at(has_dE ? c.dE : a.U0, sb, 0, pb);
+        const T Q_t = slab_at(has_Q ? c.Q : a.U0, sb, 0, pb);
+        const T W_t = HASW ? slab_at(a.W, sb, j, pb) : (T)0;
         Group<T> gr[NDIM], gc[NDIM];
-        gather_first<T, NDIM>(gr, a.U, sb, glb(a.rows), ra, pe, ro);
+        if (!HASW) gather_first<T, NDIM>(gr, a.U, sb, glb(a.rows), ra, pe, ro);
         if (FUSE) gather_first<T, NDIM>(gc, a.mu, sb, glb(a.cols), ca, pe, co);
         // W of the forward step (terms in the forward order)
-        const double w = flux_rest<T, NDIM>(gr, a.U, sb, glb(a.rows), ra, pe, ro);
+        const double w = HASW ? 0.0 : flux_rest<T, NDIM>(gr, a.U, sb, glb(a.rows), ra, pe, ro);
         double lam = (double)lam_t;
         // lambda of the launch after this one, as the transposed-flux
         // kernel stores it
         if (FUSE) lam = (double)(T)(lam + flux_rest<T, NDIM>(gc, a.mu, sb, glb(a.cols), ca, pe, co));
         const double sg = has_sig ? (double)sig_t : c.sig_s;
         const double E = has_E ? (double)E_t : c.E_s;
-        const double wq = w + (has_Q ? (double)Q_t : 0.0);
+        const double wq = HASW ? (double)W_t : w + (has_Q ? (double)Q_t : 0.0);
         const double dE = has_dE ? (double)dE_t : c.dE_s;
         // states before each application (the forward rounds to T after each)
         double u[2];
@@ -342,7 +370,13 @@ hipError_t launch_adj_update(const AdjArgs<T> &a, bool fuse, hipStream_t s) {
     const int64_t items = (rows + 3) / 4 * ((a.E[0] + 63) / 64) * a.n_grp;
     if (items >= (int64_t)1 << 31) return hipErrorInvalidValue;
     const dim3 grid((unsigned)items), block(64, 4);
-#define SMP_ADJ_UPD(F, D) hipLaunchKernelGGL((adj_update_kernel<T, F, D>), grid, block, 0, s, a)
+#define SMP_ADJ_UPD(F, D)                                                                \
+    do {                                                                                 \
+        if (a.W)                                                                         \
+            hipLaunchKernelGGL((adj_update_kernel<T, F, D, true>), grid, block, 0, s, a);  \
+        else                                                                             \
+            hipLaunchKernelGGL((adj_update_kernel<T, F, D, false>), grid, block, 0, s, a); \
+    } while (0)
     switch (a.ndim * 2 + (fuse ? 1 : 0)) {
     case 2: SMP_ADJ_UPD(false, 1); break;
     case 3: SMP_ADJ_UPD(true, 1); break;

@@ -135,6 +135,19 @@ struct smp_problem {
     int64_t adj_budget = 0;                 // bytes (0: one checkpoint per segment)
     smp::AdjSchedule adj_sched;
     int64_t adj_peak_bytes = 0;
+    // the LDS-staged adjoint update: its tile, the plans of both parities,
+    // their device records, load blobs and staged coefficient fields
+    bool adj_lds_ok = false;
+    smp::TileShape adj_ts;
+    int adj_ntile[3] = {1, 1, 1};
+    int adj_ntiles = 0;
+    smp::AdjPlan adj_plan[2];
+    void *adj_lds_recs[2] = {nullptr, nullptr}, *adj_lds_blobs[2] = {nullptr, nullptr};
+    void *adj_lds_cf[2] = {nullptr, nullptr}, *adj_lds_gf[2] = {nullptr, nullptr};
+    void *adj_zeros = nullptr;   // 64 zero dwords: boundary faces of masked mu tiles
+    int adj_lds_ncf[2] = {0, 0};
+    bool adj_lds_gen[2] = {false, false};   // some coefficient field is read from global memory
+    size_t adj_lds_bytes[2] = {0, 0};
     std::map<const void *, void *> adj_dE;   // dE/dsigma field of each sigma field
     double adj_fwd_ms = 0, adj_rev_ms = 0;
     double last_total_ms = 0, last_substep_ms = 0, last_bytes = 0;
@@ -1056,6 +1069,145 @@ double substep_bytes(const smp_problem *h, int kind) {
 // ---------------------------------------------------------------------------
 // Discrete adjoint of n time steps
 
+// The LDS-staged adjoint update (adj_lds_kernel), with SMP_ADJ_KERNEL=lds:
+// plans of both parities on the kernel's compiled-in tile, device records
+// with the coefficients of this dt, the coefficient fields staged per tile
+// (an error if the problem does not fit).  By default the gather kernels
+// run: at the 2D P15 512^2 benchmark they are faster (0.28 against 0.43 ms
+// per update launch), the LDS kernel being bound by the latency of its long
+// per-target chains at two or three cell waves per SIMD.
+template <typename T>
+int adjoint_lds_setup(smp_problem *h, const std::vector<smp::AdjComp<T>> &comp) {
+    h->adj_lds_ok = false;
+    const char *env = std::getenv("SMP_ADJ_KERNEL");
+    if (!env || std::string(env) != "lds") return SMP_OK;
+    const bool strict = true;
+    auto unavailable = [&](const std::string &why) {
+        return strict ? fail(SMP_E_INVALID, "LDS adjoint update unavailable: " + why) : SMP_OK;
+    };
+    const smp::Geometry &g = h->geo;
+    // the tile shape the kernel is compiled for
+    smp::TileShape ts;
+    if (!smp::make_tile(g, smp::adj_tile_dim(g.ndim, 0), smp::adj_tile_dim(g.ndim, 1),
+                        smp::adj_tile_dim(g.ndim, 2), ts).empty() || ts.pair)
+        return unavailable("tile shape");
+    int bpc = 1;
+    if (const char *b = std::getenv("SMP_ADJ_BLOCKS_PER_CU")) bpc = std::max(1, std::atoi(b));
+    const size_t slot_bytes = (size_t)ts.slot_elems * h->elem;
+    for (int kind = 0; kind < 2; ++kind) {
+        const std::vector<int32_t> &cs = kind == 0 ? g.odd : g.even;
+        // coefficient fields of the parity's targets
+        std::vector<const void *> table;
+        for (int j : cs)
+            for (const void *p : {(const void *)comp[j].sig, (const void *)comp[j].E,
+                                  (const void *)comp[j].dE, (const void *)comp[j].Q})
+                if (p && std::find(table.begin(), table.end(), p) == table.end()) table.push_back(p);
+        // the first MAX_CFIELDS are staged per tile, the rest read from
+        // global memory
+        if ((int)table.size() > smp::MAX_CFIELDS + 32) return unavailable("too many coefficient fields");
+        std::vector<const void *> gtable;
+        if ((int)table.size() > smp::MAX_CFIELDS) {
+            gtable.assign(table.begin() + smp::MAX_CFIELDS, table.end());
+            table.resize(smp::MAX_CFIELDS);
+        }
+        std::set<int> codes;
+        for (int j : cs) codes.insert(g.gt[3 * j] * 9 + g.gt[3 * j + 1] * 3 + g.gt[3 * j + 2]);
+        const int ngt_of = (int)codes.size();
+        if (ngt_of > 4) return unavailable("more than 4 grid types of a parity");
+        smp::AdjPlan plan;
+        std::string err = "no LDS budget";
+        for (int blocks = bpc; blocks >= 1; --blocks) {
+            const size_t per_block = (160 * 1024 - 256) / blocks;
+            const size_t fixed = smp::adj_lds_overhead((int)table.size(), ngt_of, h->elem);
+            if (per_block <= fixed + 2 * slot_bytes) continue;
+            err = smp::build_adj_plan(g, ts, kind, (int)((per_block - fixed) / slot_bytes), plan);
+            if (err.empty()) break;
+        }
+        if (!err.empty()) return unavailable(err);
+        if (plan.gcode.size() > 4) return unavailable("more than 4 grid types of a parity");
+        const int ngt = (int)plan.gcode.size();
+        const int ncf = (int)table.size();
+        const size_t base = smp::adj_lds_overhead(ncf, ngt, h->elem);
+        const int32_t slots_base = (int32_t)base;
+        auto field_off = [&](const void *p) -> int32_t {
+            if (!p) return -1;
+            const auto it = std::find(table.begin(), table.end(), p);
+            if (it != table.end()) return (int32_t)((it - table.begin()) * 256);
+            return (int32_t)(-2 - (std::find(gtable.begin(), gtable.end(), p) - gtable.begin()));
+        };
+        // device records (2 * ADJ_TPH per group; positions without a target
+        // have no points, every term in the zero tile of slot 0): offsets in
+        // bytes from the start of LDS, the coefficients of this dt
+        std::vector<smp::AdjRec> recs = plan.targets;
+        if (recs.empty()) recs.resize(1);
+        std::vector<unsigned char> blobs(std::max<size_t>(1, plan.groups.size()) * smp::BLOB_SZ, 0);
+        for (size_t q = 0; q < plan.groups.size(); ++q) {
+            const smp::DevGroup &gr = plan.groups[q];
+            unsigned char *bl = blobs.data() + q * smp::BLOB_SZ;
+            const int32_t hdr[4] = {gr.l1 - gr.l0, gr.t1 - gr.t0, 0, 0};
+            std::memcpy(bl, hdr, sizeof(hdr));
+            std::memcpy(bl + smp::ADJ_BLOB_LD_OFF, plan.loads.data() + gr.l0,
+                        sizeof(smp::DevLoad) * (gr.l1 - gr.l0));
+        }
+        for (smp::AdjRec &r : recs) {
+            for (int k = 0; k < 2 * smp::MAXT; ++k) r.off[k] = slots_base + r.off[k] * h->elem;
+            if (r.comp < 0) {
+                r.sig_off = r.E_off = r.dE_off = r.Q_off = -1;
+                continue;
+            }
+            {
+                const smp::AdjComp<T> &c = comp[r.comp];
+                r.sig_off = field_off(c.sig);
+                r.E_off = field_off(c.E);
+                r.dE_off = field_off(c.dE);
+                r.Q_off = field_off(c.Q);
+                r.sig_s = c.sig_s;
+                r.E_s = c.E_s;
+                r.dE_s = c.dE_s;
+                r.F_s[0] = c.F[0];
+                r.F_s[1] = c.F[1];
+                r.uoff = (int64_t)r.comp * h->slab * h->elem;
+            }
+        }
+        for (void **p : {&h->adj_lds_recs[kind], &h->adj_lds_blobs[kind], &h->adj_lds_cf[kind],
+                         &h->adj_lds_gf[kind]}) {
+            h->dfree(*p);
+            *p = nullptr;
+        }
+        int rc = h->dalloc(&h->adj_lds_recs[kind], recs.size() * sizeof(smp::AdjRec));
+        if (!rc) rc = h->dalloc(&h->adj_lds_blobs[kind], blobs.size());
+        if (!rc) rc = h->dalloc(&h->adj_lds_cf[kind], sizeof(void *) * smp::MAX_CFIELDS);
+        if (!rc) rc = h->dalloc(&h->adj_lds_gf[kind], sizeof(void *) * std::max<size_t>(1, gtable.size()));
+        if (rc) return rc;
+        if (!gtable.empty())
+            HIP_TRY(hipMemcpy(h->adj_lds_gf[kind], gtable.data(), sizeof(void *) * gtable.size(),
+                              hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(h->adj_lds_recs[kind], recs.data(), recs.size() * sizeof(smp::AdjRec),
+                          hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(h->adj_lds_blobs[kind], blobs.data(), blobs.size(), hipMemcpyHostToDevice));
+        if (!table.empty())
+            HIP_TRY(hipMemcpy(h->adj_lds_cf[kind], table.data(), sizeof(void *) * table.size(),
+                              hipMemcpyHostToDevice));
+        h->adj_plan[kind] = plan;
+        h->adj_lds_ncf[kind] = ncf;
+        h->adj_lds_gen[kind] = !gtable.empty();
+        h->adj_lds_bytes[kind] = base + (size_t)std::max(plan.n_slots, 1) * slot_bytes;
+    }
+    if (!h->adj_zeros) {
+        int rc = h->dalloc(&h->adj_zeros, 256);
+        if (rc) return rc;
+        HIP_TRY(hipMemset(h->adj_zeros, 0, 256));
+    }
+    h->adj_ts = ts;
+    h->adj_ntiles = 1;
+    for (int e = 0; e < 3; ++e) {
+        h->adj_ntile[e] = (h->E[e] + ts.tile[e] - 1) / ts.tile[e];
+        h->adj_ntiles *= h->adj_ntile[e];
+    }
+    h->adj_lds_ok = true;
+    return SMP_OK;
+}
+
 // device tables of the adjoint kernels (coefficients as the forward uses
 // them at the current dt)
 template <typename T>
@@ -1200,7 +1352,7 @@ int adjoint_tables(smp_problem *h, double dt) {
         HIP_TRY(hipMemcpy(h->adj_cols, cols.data(), sizeof(smp::AdjTerm) * cols.size(),
                           hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(h->adj_list, list.data(), sizeof(int32_t) * list.size(), hipMemcpyHostToDevice));
-    return SMP_OK;
+    return adjoint_lds_setup<T>(h, comp);
 }
 
 int adjoint_buffers(smp_problem *h) {
@@ -1243,13 +1395,24 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
     rc = adjoint_tables<T>(h, dt);
     if (rc) return rc;
     const size_t st = (size_t)h->slab * h->elem * h->geo.n_sys;
+    // Recording launches also store W+Q of their targets, so that the
+    // gather update needs no forward flux of its own: 16 fewer gathers per
+    // target in 2D, about 25% less time per update launch, for twice the
+    // memory per recorded segment (under a budget, fewer segments recorded
+    // in the forward pass).  Not with global-memory coefficients, with the
+    // LDS update (which stages the sources anyway) or with
+    // SMP_ADJ_RECORD_W=0.
+    const char *rw_env = std::getenv("SMP_ADJ_RECORD_W");
+    const char *kn_env = std::getenv("SMP_ADJ_KERNEL");
+    const bool recW = !h->gen[0] && !h->gen[1] && !(rw_env && std::string(rw_env) == "0") &&
+                      !(kn_env && std::string(kn_env) == "lds");
     smp::AdjSchedule sc;
     {
         const int64_t nbuf = h->adj_budget > 0 ? h->adj_budget / (int64_t)st : -1;
-        const std::string err = smp::plan_adjoint(n_steps, every, nbuf, sc);
+        const std::string err = smp::plan_adjoint(n_steps, every, nbuf, recW ? 2 : 1, sc);
         if (!err.empty()) return fail(h->adj_budget > 0 ? SMP_E_OOM : SMP_E_INVALID, err);
     }
-    const int64_t k = sc.seg_steps, rb = k + 1;
+    const int64_t k = sc.seg_steps, rb = sc.rec_factor * (k + 1);
     const int H = sc.segments - sc.tail;
     // buffers of the schedule, kept on the handle for the next call
     while ((int)h->adj_slot.size() < sc.slots) {
@@ -1310,6 +1473,10 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
     auto rec = [&](int r, int64_t L) -> void * {
         return (char *)h->adj_range[r] + (size_t)(L / 2) * st;
     };
+    // W+Q of launch L's targets, in the second half of the range
+    auto wrec = [&](int r, int64_t L) -> void * {
+        return recW ? (char *)h->adj_range[r] + (size_t)(k + 1 + L / 2) * st : nullptr;
+    };
     auto buf_of = [&](int id) -> void * {
         if (id >= 0) return h->adj_slot[id];
         if (id == smp::ADJ_WORK) return h->adj_work;
@@ -1346,6 +1513,7 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
             a.U = (T *)after(s0, r, L - 2);
             a.src = (const T *)after(s0, r, L - 1);
             a.out = (T *)rec(r, L);
+            a.wout = (T *)wrec(r, L);
             HIP_TRY(smp::launch_substep<T>(a, h->plan[kind].max_terms, h->gen[kind], h->cell_waves,
                                            h->lds_bytes[kind], strm));
         }
@@ -1369,8 +1537,8 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
     const int nl_odd = (int)h->geo.odd.size(), nl_even = (int)h->geo.even.size();
     const int32_t *list_odd = (const int32_t *)h->adj_list;
     const int32_t *list_even = list_odd + nl_odd;
-    auto adj_args = [&](const void *U, const void *U0, const int32_t *list, int nl, int napp, int f0,
-                        int f1) {
+    auto adj_args = [&](const void *U, const void *U0, const void *W, const int32_t *list, int nl,
+                        int napp, int f0, int f1) {
         const int par = list == list_odd ? 0 : 1;
         smp::AdjArgs<T> a;
         std::memset(&a, 0, sizeof(a));
@@ -1384,6 +1552,7 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
         for (int q = 0; q < 8; ++q) a.gcode[q] = h->adj_gcode[par][q];
         a.U = (const T *)U;
         a.U0 = (const T *)U0;
+        a.W = (const T *)W;
         a.lam = (T *)h->lam;
         a.mu = (T *)h->mu;
         a.Ga = (double *)h->G;
@@ -1403,6 +1572,57 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
         a.ndim = h->geo.ndim;
         return a;
     };
+    auto lds_args = [&](const void *U, const void *U0, int kind, int napp, int f0, int f1) {
+        smp::AdjTileArgs<T> a;
+        std::memset(&a, 0, sizeof(a));
+        const smp::TileShape &ts = h->adj_ts;
+        const smp::AdjPlan &p = h->adj_plan[kind];
+        a.U = (const T *)U;
+        a.U0 = (const T *)U0;
+        a.lam = (T *)h->lam;
+        a.mu = (T *)h->mu;
+        a.Ga = (double *)h->G;
+        a.Gs = (double *)h->G + 8 * h->slab;
+        a.sink = (T *)h->sink;
+        a.slab = h->slab;
+        a.pitch0 = h->pitch0;
+        a.ext1 = h->E[1];
+        for (int e = 0; e < 3; ++e) {
+            a.n[e] = e < h->geo.ndim ? h->geo.n[h->geo.axis[e]] : 1;
+            a.bc[e] = e < h->geo.ndim ? h->geo.bc[h->geo.axis[e]] : 0;
+            a.tile[e] = ts.tile[e];
+            a.ntile[e] = h->adj_ntile[e];
+            a.lds[e] = ts.lds[e];
+            a.E[e] = h->E[e];
+        }
+        a.slot_elems = ts.slot_elems;
+        while ((1 << a.lds0_shift) < ts.lds[0]) ++a.lds0_shift;
+        a.inv_lds1 = 1.0f / (float)ts.lds[1];
+        a.chunks = (ts.slot_elems * (h->elem / 4) + 63) / 64;
+        a.n_tiles = h->adj_ntiles;
+        a.ngroups = (int)p.groups.size();
+        a.blobs = (const unsigned char *)h->adj_lds_blobs[kind];
+        a.recs = (const smp::AdjRec *)h->adj_lds_recs[kind];
+        a.zeros = (const unsigned *)h->adj_zeros;
+        a.cfields = (const void *const *)h->adj_lds_cf[kind];
+        a.gfields = (const void *const *)h->adj_lds_gf[kind];
+        a.ncf = h->adj_lds_ncf[kind];
+        a.half_dt = dt / 2;
+        a.napp = napp;
+        a.fslot[0] = f0;
+        a.fslot[1] = f1;
+        a.ngt = (int)p.gcode.size();
+        for (int q = 0; q < a.ngt; ++q) {
+            const int code = p.gcode[q];
+            a.gcode[q] = code;
+            const int gt[3] = {code / 4 + 1, (code / 2) % 2 + 1, code % 2 + 1};
+            for (int e = 0; e < 3; ++e) {
+                const int d = e < h->geo.ndim ? h->geo.axis[e] : -1;
+                a.gext[q][e] = d < 0 ? 1 : (gt[d] == 1 ? h->geo.n[d] : h->geo.n[d] + h->geo.bc[d]);
+            }
+        }
+        return a;
+    };
     // the adjoint of segment sg, launch by launch from the last: the update
     // of L's targets (L's targets before L and its sources are the states
     // after launches L-2 and L-1), with the transposed flux of launch L+1
@@ -1417,10 +1637,16 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
             const void *U = after(s0, r, L - 1);
             const int32_t *tl = kind ? list_even : list_odd, *sl = kind ? list_odd : list_even;
             const int ntl = kind ? nl_even : nl_odd, nsl = kind ? nl_odd : nl_even;
-            HIP_TRY(smp::launch_adj_update<T>(adj_args(U, U0, tl, ntl, f.napp, f.fslot[0], f.fslot[1]),
-                                              L + 1 < nl, h->stream));
+            if (h->adj_lds_ok)
+                HIP_TRY(smp::launch_adj_lds<T>(lds_args(U, U0, kind, f.napp, f.fslot[0], f.fslot[1]),
+                                               h->geo.ndim, L + 1 < nl, h->adj_lds_gen[kind],
+                                               h->adj_lds_bytes[kind], h->stream));
+            else
+                HIP_TRY(smp::launch_adj_update<T>(adj_args(U, U0, wrec(r, L), tl, ntl, f.napp, f.fslot[0],
+                                                           f.fslot[1]),
+                                                  L + 1 < nl, h->stream));
             if (L == 0)
-                HIP_TRY(smp::launch_adj_transpose<T>(adj_args(U, U0, sl, nsl, f.napp, f.fslot[0],
+                HIP_TRY(smp::launch_adj_transpose<T>(adj_args(U, U0, nullptr, sl, nsl, f.napp, f.fslot[0],
                                                               f.fslot[1]),
                                                      h->stream));
         }
@@ -2288,11 +2514,12 @@ int smp_adjoint_stats(const smp_problem *h, int64_t stats[8]) {
     return SMP_OK;
 }
 
-int smp_adjoint_schedule(int64_t n_steps, int32_t every, int64_t buffers, int64_t stats[8],
-                         int32_t *actions, int64_t cap, int64_t *n_fwd, int64_t *n_actions) {
+int smp_adjoint_schedule(int64_t n_steps, int32_t every, int64_t buffers, int32_t rec_factor,
+                         int64_t stats[8], int32_t *actions, int64_t cap, int64_t *n_fwd,
+                         int64_t *n_actions) {
     if (!stats || !n_fwd || !n_actions) return fail(SMP_E_INVALID, "null argument");
     smp::AdjSchedule sc;
-    const std::string err = smp::plan_adjoint(n_steps, every, buffers, sc);
+    const std::string err = smp::plan_adjoint(n_steps, every, buffers, rec_factor, sc);
     if (!err.empty()) return fail(buffers >= 0 ? SMP_E_OOM : SMP_E_INVALID, err);
     adjoint_stats_of(sc, 0, stats);
     *n_fwd = (int64_t)sc.fwd.size();

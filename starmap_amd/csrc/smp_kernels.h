@@ -53,6 +53,8 @@ struct SubstepArgs {
     int32_t napp;             // half-step applications with the same flux (1 or 2)
     int32_t fslot[2];         // filter slot of application 0 and 1
     int32_t pair;             // two cells per cell thread (tiles of more than 256 cells)
+    T *wout;                  // if not null: W+Q of every target and point (state layout; the
+                              // discrete adjoint's record of the launch), without GEN coefficients
 };
 
 // One problem's substep in a batched launch: its arguments and the index of
@@ -208,6 +210,8 @@ struct AdjArgs {
     int32_t gcode[8];         // grid type code (gtx-1)*4+(gty-1)*2+(gtz-1)
     const T *U;               // forward state during the substep (sources)
     const T *U0;              // targets before the substep (rows kernel)
+    const T *W;               // W+Q of the targets as the forward launch stored it, or null
+                              // (the update kernel then computes the flux W itself)
     T *lam;                   // adjoint state
     T *mu;                    // adjoint of the flux W of the targets
     double *Ga, *Gs;          // dJ/dsigma_a, dJ/dsigma_s0 per grid type code
@@ -243,5 +247,56 @@ hipError_t launch_adj_transpose(const AdjArgs<T> &a, hipStream_t s);
 // dst[i] = src[i] for a contiguous range of elements
 template <typename T>
 hipError_t launch_copy(T *dst, const T *src, int64_t count, hipStream_t s);
+
+// ---- the LDS-staged adjoint update (one launch's targets) -----------------
+// Arguments of adj_lds_kernel: the tile decomposition and LDS layout of the
+// substep kernel (one cell per cell thread, 4 cell waves beside 4 copy
+// waves), the plan of one parity (build_adj_plan), and per launch the
+// forward states and napp/filter slots of the launch it reverses.
+template <typename T>
+struct AdjTileArgs {
+    const T *U;               // the launch's sources (the state after launch L-1)
+    const T *U0;              // its targets before it (after launch L-2)
+    T *lam, *mu;              // adjoint state and adjoint of the flux (state layout)
+    double *Ga, *Gs;          // dJ/dsigma_a, dJ/dsigma_s0: 8 grid-type slabs each
+    T *sink;                  // 1024 elements written by lanes without a target point
+    int64_t slab;
+    int32_t pitch0, ext1;
+    int32_t n[3], bc[3];
+    int32_t tile[3], ntile[3], lds[3];
+    int32_t slot_elems, lds0_shift;
+    float inv_lds1;
+    int32_t chunks, n_tiles, ngroups;
+    const unsigned char *blobs;   // per group: header {loads, targets} and load records
+    const AdjRec *recs;           // ADJ_RECS records per group
+    const unsigned *zeros;        // 64 zero dwords (boundary faces of masked mu tiles)
+    const void *const *cfields;   // coefficient fields staged per tile
+    int32_t ncf;
+    const void *const *gfields;   // coefficient fields read from global memory (off -2-k: field k)
+    int32_t E[3];
+    double half_dt;
+    int32_t napp, fslot[2];
+    int32_t ngt;                  // grid types of the targets, their codes and extents
+    int32_t gcode[4];
+    int32_t gext[4][3];
+};
+
+// LDS bytes before the source slots: blob ring, staged fields, gradient
+// accumulators of ngt grid types
+size_t adj_lds_overhead(int ncf, int ngt, int elem);
+constexpr int ADJ_THREADS = 1024;
+// the tile shape the kernel is compiled for, per number of kernel axes
+// (cells per axis; the LDS slot adds a one-cell halo on each axis)
+constexpr int adj_tile_dim(int ndim, int e) {
+    return ndim == 1 ? (e == 0 ? 255 : 1) : ndim == 2 ? (e == 0 ? 63 : e == 1 ? 4 : 1)
+                                                       : (e == 0 ? 31 : e == 1 ? 4 : 2);
+}
+
+// ndim: active kernel axes; fuse: add the transposed flux of the launch
+// after first (mu tiles staged); gen: some coefficient is a field read from
+// global memory (more fields than the LDS stages)
+template <typename T>
+hipError_t launch_adj_lds(const AdjTileArgs<T> &a, int ndim, bool fuse, bool gen, size_t lds_bytes,
+                          hipStream_t s);
 
 }  // namespace smp

@@ -87,7 +87,10 @@ __device__ __forceinline__ SMP_GLOBAL P *glb(P *p) {
 // and two LDS reads of neighbouring elements for the two cells, and a target
 // record read serves both; each cell's arithmetic is that of the one-cell
 // path, so the results agree bit for bit.
-template <typename T, int NT, int NAPP, bool GEN, int CW, int NCW, bool WARM, bool PAIR, typename AT>
+// WOUT: also store W+Q of every target and point to a.wout (for the discrete
+// adjoint, which then needs no flux of its own for the gradient).
+template <typename T, int NT, int NAPP, bool GEN, int CW, int NCW, bool WARM, bool PAIR, bool WOUT,
+          typename AT>
 __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsigned char *smem_raw) {
     constexpr int NCT = CW * 64;              // cell threads
     constexpr int TPT = GMAX * 4 / CW;        // targets of a group per cell thread
@@ -280,6 +283,7 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     // outside the storage (the value is not used)
     const int d1 = in_store1 ? (int)sizeof(T) : 0;
     SMP_GLOBAL T *sink = glb(a.sink);
+    SMP_GLOBAL char *wo_st = WOUT ? glb(reinterpret_cast<char *>(a.wout)) + gidx * sizeof(T) : nullptr;
     // The cell waves' records of group gq: with SMP_SCALAR_RECORDS, read
     // with scalar loads from the plan in global memory (every workgroup
     // reads the same records, which stay in the scalar cache and L2);
@@ -358,6 +362,11 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
                 const D sig = coef(r.sig_off, r.sig_p, r.sig_s, k);
                 const D E = coef(r.E_off, r.E_p, r.E_s, k);
                 const D wq = (D)w[k] + coef(r.Q_off, r.Q_p, 0.0, k);
+                if (WOUT) {
+                    SMP_GLOBAL T *dw = ak ? reinterpret_cast<SMP_GLOBAL T *>(wo_st + r.uoff + k * sizeof(T))
+                                          : sink + PC * tid + k;
+                    *dw = (T)wq;
+                }
                 D u = (D)u_in[PC * i + k];
 #pragma unroll
                 for (int app = 0; app < NAPP; ++app) {
@@ -460,12 +469,12 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     }
 }
 
-template <typename T, int NT, int NAPP, bool GEN, int CW, bool PAIR>
+template <typename T, int NT, int NAPP, bool GEN, int CW, bool PAIR, bool WOUT>
 __global__ __launch_bounds__(kthreads(CW)) void substep_kernel(const SubstepArgs<T> a) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= a.n_tiles) return;
-    substep_tile<T, NT, NAPP, GEN, CW, copy_waves(CW), false, PAIR>(a, tile, smem_raw);
+    substep_tile<T, NT, NAPP, GEN, CW, copy_waves(CW), false, PAIR, WOUT>(a, tile, smem_raw);
 }
 
 // Batched persistent kernel: one launch runs a substep of many independent
@@ -510,8 +519,8 @@ __global__ __launch_bounds__(batch_threads(CW)) void batch_kernel(const BatchLau
         // the launch's application count is compiled in (a runtime count
         // costs the update its instruction schedule, and both tile paths in
         // one kernel outgrow the instruction cache)
-        substep_tile<T, NT, NAPP, GEN, CW, batch_copy_waves(CW), true, PAIR>(a, u - ent[lo].unit0,
-                                                                               smem_raw);
+        substep_tile<T, NT, NAPP, GEN, CW, batch_copy_waves(CW), true, PAIR, false>(a, u - ent[lo].unit0,
+                                                                                      smem_raw);
     }
 }
 
@@ -520,24 +529,30 @@ __global__ void fill_kernel(int *p, int n, int v) {
     if (i < n) p[i] = v;
 }
 
-template <typename T, int NT, int NAPP, bool GEN, int CW, bool PAIR>
+template <typename T, int NT, int NAPP, bool GEN, int CW, bool PAIR, bool WOUT = false>
 static hipError_t launch_v(const SubstepArgs<T> &a, size_t lds_bytes, hipStream_t s) {
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void *)substep_kernel<T, NT, NAPP, GEN, CW, PAIR>,
+        hipError_t e = hipFuncSetAttribute((const void *)substep_kernel<T, NT, NAPP, GEN, CW, PAIR, WOUT>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize,
                                            160 * 1024);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
     const int nb = ((a.n_tiles + 7) / 8) * 8;
-    hipLaunchKernelGGL((substep_kernel<T, NT, NAPP, GEN, CW, PAIR>), dim3(nb), dim3(kthreads(CW)),
+    hipLaunchKernelGGL((substep_kernel<T, NT, NAPP, GEN, CW, PAIR, WOUT>), dim3(nb), dim3(kthreads(CW)),
                        lds_bytes, s, a);
     return hipGetLastError();
 }
 
 template <typename T, int NT, int CW, bool PAIR>
 static hipError_t launch_nt(const SubstepArgs<T> &a, bool gen, size_t lds_bytes, hipStream_t s) {
+    if (a.wout) {
+        // W+Q output: the variants without global-memory coefficients only
+        if (gen) return hipErrorInvalidValue;
+        return a.napp == 1 ? launch_v<T, NT, 1, false, CW, PAIR, true>(a, lds_bytes, s)
+                           : launch_v<T, NT, 2, false, CW, PAIR, true>(a, lds_bytes, s);
+    }
     if (a.napp == 1)
         return gen ? launch_v<T, NT, 1, true, CW, PAIR>(a, lds_bytes, s)
                    : launch_v<T, NT, 1, false, CW, PAIR>(a, lds_bytes, s);

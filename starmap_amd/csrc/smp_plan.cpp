@@ -121,6 +121,121 @@ TileShape choose_tile(const Geometry &g, int elem_bytes) {
     return ts;
 }
 
+namespace {
+
+// One group of a slot schedule: targets [t0, t1), the source tiles it
+// stages (source id, slot), and the slot of each of its sources (ids sorted)
+struct SlotGroup {
+    int t0 = 0, t1 = 0;
+    std::vector<std::pair<int, int>> loads;
+    std::vector<int> ids, slots;
+    int slot(int id) const {
+        const auto it = std::lower_bound(ids.begin(), ids.end(), id);
+        return slots[it - ids.begin()];
+    }
+};
+
+// Groups of consecutive targets and their LDS slots.  A group's tiles are
+// copied in while the previous group is still read, so two consecutive
+// groups must fit together: each group gets half of the slots.  Slot 0 is a
+// zero tile (padding terms read it).  A new tile takes a free slot, or
+// evicts the source whose next use is farthest away, never one the current
+// or the previous group reads.
+// fixed_groups: every group takes exactly group_size consecutive targets
+// (an error if their sources do not fit)
+std::string schedule_slots(const std::vector<std::vector<int>> &src, int n_ids, int slot_capacity,
+                           int group_size, std::vector<SlotGroup> &out, int &used,
+                           bool fixed_groups = false) {
+    out.clear();
+    used = 0;
+    if (slot_capacity < 2) return "too few LDS slots";
+    const int group_cap = (slot_capacity - 1) / 2;
+    std::vector<std::vector<int>> gsrc;
+    for (size_t t = 0; t < src.size();) {
+        size_t t1 = t;
+        std::set<int> s;
+        while (t1 < src.size() && (int)(t1 - t) < group_size) {
+            std::set<int> s2 = s;
+            s2.insert(src[t1].begin(), src[t1].end());
+            if ((int)s2.size() > group_cap) break;
+            s = s2;
+            ++t1;
+        }
+        if (t1 == t) return "a target couples to more components than half the LDS slots";
+        if (fixed_groups && (int)(t1 - t) < group_size && t1 < src.size())
+            return "a group's sources do not fit in half the LDS slots";
+        SlotGroup gr;
+        gr.t0 = (int)t;
+        gr.t1 = (int)t1;
+        out.push_back(gr);
+        gsrc.push_back(std::vector<int>(s.begin(), s.end()));
+        t = t1;
+    }
+    const int INF = std::numeric_limits<int>::max();
+    std::vector<int> slot_of(n_ids, -1);
+    std::vector<int> owner(slot_capacity, -1);
+    owner[0] = -2;
+    auto next_use = [&](int k, size_t from) {
+        for (size_t q = from; q < gsrc.size(); ++q)
+            if (std::binary_search(gsrc[q].begin(), gsrc[q].end(), k)) return (int)q;
+        return INF;
+    };
+    for (size_t q = 0; q < out.size(); ++q) {
+        for (int k : gsrc[q]) {
+            if (slot_of[k] >= 0) continue;
+            int s = -1;
+            for (int c = 1; c < slot_capacity; ++c)
+                if (owner[c] == -1) { s = c; break; }
+            if (s < 0) {
+                int best = -1, far = -1;
+                for (int c = 1; c < slot_capacity; ++c) {
+                    int o = owner[c];
+                    if (std::binary_search(gsrc[q].begin(), gsrc[q].end(), o)) continue;
+                    // the previous group still reads its sources while this
+                    // group's tiles are copied in
+                    if (q > 0 && std::binary_search(gsrc[q - 1].begin(), gsrc[q - 1].end(), o))
+                        continue;
+                    int nu = next_use(o, q + 1);
+                    if (nu > far) { far = nu; best = c; }
+                }
+                if (best < 0) return "LDS slot assignment failed";
+                slot_of[owner[best]] = -1;
+                s = best;
+            }
+            owner[s] = k;
+            slot_of[k] = s;
+            used = std::max(used, s + 1);
+            out[q].loads.push_back({k, s});
+        }
+        out[q].ids = gsrc[q];
+        for (int k : gsrc[q]) out[q].slots.push_back(slot_of[k]);
+    }
+    return "";
+}
+
+// LDS element offset of the low sample of a term of a target from source
+// tile `slot` of a component of grid type gts along kernel axis kd: the
+// slot base, shifted by one on every other axis where the source is
+// cell-centred (its slot starts one cell lower)
+int term_offset(const Geometry &g, const TileShape &ts, int slot, const int32_t *gts, int kd) {
+    const int stride[3] = {1, ts.lds[0], ts.lds[0] * ts.lds[1]};
+    int off = slot * ts.slot_elems;
+    for (int e = 0; e < g.ndim; ++e)
+        if (e != kd && gts[g.axis[e]] == 1) off += stride[e];
+    return off;
+}
+
+DevLoad load_record(const Geometry &g, int comp, int slot) {
+    DevLoad ld;
+    std::memset(&ld, 0, sizeof(ld));
+    ld.comp = comp;
+    ld.slot = slot;
+    for (int e = 0; e < 3; ++e) ld.org[e] = e < g.ndim && g.gt[3 * comp + g.axis[e]] == 1 ? -1 : 0;
+    return ld;
+}
+
+}  // namespace
+
 std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slot_capacity,
                        int group_size, int max_load_elems, int max_loads_per_group, Plan &out) {
     out = Plan();
@@ -150,76 +265,13 @@ std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slo
     }
     out.n_sources = (int)all_src.size();
 
-    // groups of consecutive targets; a group's tiles are copied in while the
-    // previous group is still read, so two consecutive groups must fit
-    // together: each group gets half of the slots
-    const int group_cap = (slot_capacity - 1) / 2;
-    std::vector<std::pair<int, int>> groups;
-    std::vector<std::vector<int>> gsrc;
-    for (size_t t = 0; t < tg.size();) {
-        size_t t1 = t;
-        std::set<int> s;
-        while (t1 < tg.size() && (int)(t1 - t) < group_size) {
-            std::set<int> s2 = s;
-            s2.insert(src[t1].begin(), src[t1].end());
-            if ((int)s2.size() > group_cap) break;
-            s = s2;
-            ++t1;
-        }
-        if (t1 == t) return "a target couples to more components than half the LDS slots";
-        groups.push_back({(int)t, (int)t1});
-        gsrc.push_back(std::vector<int>(s.begin(), s.end()));
-        t = t1;
-    }
-
-    // slot assignment: keep what the next groups need, evict the source
-    // whose next use is farthest away
-    const int INF = std::numeric_limits<int>::max();
-    std::vector<int> slot_of(g.n_sys, -1);
-    // slot 0 is a zero tile: padding terms read it
-    std::vector<int> owner(slot_capacity, -1);
-    if (slot_capacity < 2) return "too few LDS slots";
-    owner[0] = -2;
-    auto next_use = [&](int k, size_t from) {
-        for (size_t q = from; q < gsrc.size(); ++q)
-            if (std::binary_search(gsrc[q].begin(), gsrc[q].end(), k)) return (int)q;
-        return INF;
-    };
+    std::vector<SlotGroup> sg;
     int used = 0;
-    for (size_t q = 0; q < groups.size(); ++q) {
+    std::string err = schedule_slots(src, g.n_sys, slot_capacity, group_size, sg, used);
+    if (!err.empty()) return err;
+    for (const SlotGroup &q : sg) {
         std::vector<DevLoad> new_loads;
-        for (int k : gsrc[q]) {
-            if (slot_of[k] >= 0) continue;
-            int s = -1;
-            for (int c = 1; c < slot_capacity; ++c)
-                if (owner[c] == -1) { s = c; break; }
-            if (s < 0) {
-                int best = -1, far = -1;
-                for (int c = 1; c < slot_capacity; ++c) {
-                    int o = owner[c];
-                    if (std::binary_search(gsrc[q].begin(), gsrc[q].end(), o)) continue;
-                    // the previous group still reads its sources while this
-                    // group's tiles are copied in
-                    if (q > 0 && std::binary_search(gsrc[q - 1].begin(), gsrc[q - 1].end(), o))
-                        continue;
-                    int nu = next_use(o, q + 1);
-                    if (nu > far) { far = nu; best = c; }
-                }
-                if (best < 0) return "LDS slot assignment failed";
-                slot_of[owner[best]] = -1;
-                s = best;
-            }
-            owner[s] = k;
-            slot_of[k] = s;
-            used = std::max(used, s + 1);
-            DevLoad ld;
-            std::memset(&ld, 0, sizeof(ld));
-            ld.comp = k;
-            ld.slot = s;
-            for (int e = 0; e < 3; ++e)
-                ld.org[e] = e < g.ndim && g.gt[3 * k + g.axis[e]] == 1 ? -1 : 0;
-            new_loads.push_back(ld);
-        }
+        for (const auto &ks : q.loads) new_loads.push_back(load_record(g, ks.first, ks.second));
         // staging is bounded by the prefetch registers: leading load-only
         // groups take what does not fit
         size_t first = 0;
@@ -238,7 +290,7 @@ std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slo
         out.loads.insert(out.loads.end(), new_loads.begin() + first, new_loads.end());
         gr.l1 = (int)out.loads.size();
         gr.t0 = (int)out.targets.size();
-        for (int t = groups[q].first; t < groups[q].second; ++t) {
+        for (int t = q.t0; t < q.t1; ++t) {
             int j = tg[t];
             TgtRec r;
             std::memset(&r, 0, sizeof(r));
@@ -254,10 +306,7 @@ std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slo
                 for (int p = M.rp[j]; p < M.rp[j + 1]; ++p) {
                     int k = M.ci[p];
                     if (kd < 0) return "coupling along an axis without cells";
-                    int off = slot_of[k] * ts.slot_elems;
-                    for (int e = 0; e < g.ndim; ++e)
-                        if (e != kd && g.gt[3 * k + g.axis[e]] == 1) off += stride[e];
-                    r.t[nt].off = off;
+                    r.t[nt].off = term_offset(g, ts, q.slot(k), &g.gt[3 * k], kd);
                     r.t[nt].stride = stride[kd];
                     r.t[nt].coef = -M.cv[p] / g.h[d];
                     ++nt;
@@ -265,15 +314,137 @@ std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slo
             }
             // padding terms (coefficient 0 times a difference within the zero
             // tile of slot 0) keep the kernel's term loop free of branches
-            for (int q = nt; q < MAXT; ++q) {
-                r.t[q].off = 0;
-                r.t[q].stride = 0;
-                r.t[q].coef = 0.0;
+            for (int p = nt; p < MAXT; ++p) {
+                r.t[p].off = 0;
+                r.t[p].stride = 0;
+                r.t[p].coef = 0.0;
             }
             out.targets.push_back(r);
         }
         gr.t1 = (int)out.targets.size();
         out.groups.push_back(gr);
+    }
+    out.n_slots = used;
+    return "";
+}
+
+std::string build_adj_plan(const Geometry &g, const TileShape &ts, int kind, int slot_capacity,
+                           AdjPlan &out) {
+    out = AdjPlan();
+    if (ts.pair) return "the adjoint update takes one cell per thread";
+    if (ADJ_TERMS * g.ndim > MAXT) return "too many kernel axes";
+    const int n = g.n_sys;
+    int kaxis_of[3] = {-1, -1, -1};
+    for (int e = 0; e < g.ndim; ++e) kaxis_of[g.axis[e]] = e;
+    // targets of the parity grouped by grid type (codes in order of first
+    // appearance), components ascending within a grid type
+    const std::vector<int32_t> &cs = kind == 0 ? g.odd : g.even;
+    auto code_of = [&](int j) { return (g.gt[3 * j] - 1) * 4 + (g.gt[3 * j + 1] - 1) * 2 + (g.gt[3 * j + 2] - 1); };
+    for (int j : cs)
+        if (std::find(out.gcode.begin(), out.gcode.end(), code_of(j)) == out.gcode.end())
+            out.gcode.push_back(code_of(j));
+    std::vector<int> tg;
+    for (int c : out.gcode)
+        for (int j : cs)
+            if (code_of(j) == c) tg.push_back(j);
+    // transposed coupling lists: rows i with M_d(i, j) != 0, ascending i
+    std::vector<std::vector<std::vector<std::pair<int, double>>>> cols(
+        3, std::vector<std::vector<std::pair<int, double>>>(n));
+    for (int d = 0; d < 3; ++d) {
+        const Csr &M = g.M[d];
+        if (M.rp.empty()) continue;
+        for (int i = 0; i < n; ++i)
+            for (int p = M.rp[i]; p < M.rp[i + 1]; ++p) cols[d][M.ci[p]].push_back({i, M.cv[p]});
+    }
+    // source ids: k the state tile of component k; n*(1+m) + i the mu tile
+    // of component i, m = 0 plain, m = 1+e with the boundary faces of kernel
+    // axis e zero (a coupling along e to a target cell-centred on e)
+    auto mu_id = [&](int j, int i, int e) {
+        const bool masked = g.bc[g.axis[e]] == 1 && g.gt[3 * j + g.axis[e]] == 1;
+        return n * (1 + (masked ? 1 + e : 0)) + i;
+    };
+    std::vector<std::vector<int>> src(tg.size());
+    for (size_t t = 0; t < tg.size(); ++t) {
+        const int j = tg[t];
+        std::set<int> s;
+        for (int d = 0; d < 3; ++d) {
+            const Csr &M = g.M[d];
+            if (M.rp.empty()) continue;
+            const int e = kaxis_of[d];
+            if (e < 0) return "coupling along an axis without cells";
+            if (M.rp[j + 1] - M.rp[j] > ADJ_TERMS || (int)cols[d][j].size() > ADJ_TERMS)
+                return "more than 4 couplings of a component along an axis";
+            for (int p = M.rp[j]; p < M.rp[j + 1]; ++p) s.insert(M.ci[p]);
+            for (const auto &ic : cols[d][j]) s.insert(mu_id(j, ic.first, e));
+        }
+        src[t].assign(s.begin(), s.end());
+    }
+    std::vector<SlotGroup> sg;
+    int used = 0;
+    std::string err = schedule_slots(src, 5 * n, slot_capacity, ADJ_RECS, sg, used);
+    if (!err.empty()) return err;
+    // the record positions of each group: the group's targets in order
+    // (positions p*ADJ_TPH .. (p+1)*ADJ_TPH-1 for part p of the cell
+    // threads), then positions without a target
+    std::vector<int> pos_of;   // record index of every target
+    for (const SlotGroup &q : sg) {
+        if ((int)q.loads.size() > ADJ_MAXL) return "more than 64 source tiles in a group";
+        DevGroup gr;
+        gr.l0 = (int)out.loads.size();
+        for (const auto &ks : q.loads) {
+            const int mode = ks.first / n;   // 0 state, 1 mu, 2+e mu masked on e
+            DevLoad ld = load_record(g, ks.first % n, ks.second);
+            ld.pad[0] = mode;
+            out.loads.push_back(ld);
+        }
+        gr.l1 = (int)out.loads.size();
+        gr.t0 = (int)out.targets.size();
+        for (int p = 0; p < ADJ_RECS; ++p) {
+            AdjRec r;
+            std::memset(&r, 0, sizeof(r));
+            r.comp = -1;
+            const int t = q.t0 + p;
+            if (t < q.t1) {
+                const int j = tg[t];
+                r.comp = j;
+                r.gq = (int32_t)(std::find(out.gcode.begin(), out.gcode.end(), code_of(j)) - out.gcode.begin());
+                r.jpos = j > 0;
+                for (int e = 0; e < 3; ++e) r.ext[e] = e < g.ndim ? g.ext_of(j, g.axis[e]) : 1;
+                for (int e = 0; e < g.ndim; ++e) {
+                    const int d = g.axis[e];
+                    const Csr &M = g.M[d];
+                    if (M.rp.empty()) continue;
+                    int nf = 0, nb = 0;
+                    for (int pp = M.rp[j]; pp < M.rp[j + 1]; ++pp, ++nf) {
+                        const int k = M.ci[pp];
+                        r.off[ADJ_TERMS * e + nf] = term_offset(g, ts, q.slot(k), &g.gt[3 * k], e);
+                        r.coef[ADJ_TERMS * e + nf] = -M.cv[pp] / g.h[d];
+                    }
+                    // lambda_j += sum_i c_ij (D^T mu_i): the forward-form
+                    // difference of mu_i at j's points, coefficient -c_ij
+                    for (const auto &ic : cols[d][j]) {
+                        const int i = ic.first, qq = MAXT + ADJ_TERMS * e + nb++;
+                        r.off[qq] = term_offset(g, ts, q.slot(mu_id(j, i, e)), &g.gt[3 * i], e);
+                        r.coef[qq] = ic.second / g.h[d];
+                    }
+                }
+                pos_of.push_back((int)out.targets.size());
+            }
+            out.targets.push_back(r);
+        }
+        gr.t1 = (int)out.targets.size();
+        out.groups.push_back(gr);
+    }
+    // each part sums the gradient of a grid type over its targets of that
+    // grid type in order and stores the sum after the last of them
+    for (int h = 0; h < ADJ_PARTS; ++h) {
+        int last = -1;
+        for (int p : pos_of) {
+            if ((p % ADJ_RECS) / ADJ_TPH != h) continue;
+            if (last >= 0 && out.targets[last].gq != out.targets[p].gq) out.targets[last].flush = 1;
+            last = p;
+        }
+        if (last >= 0) out.targets[last].flush = 1;
     }
     out.n_slots = used;
     return "";
@@ -404,15 +575,18 @@ struct AdjEmitter {
 
 }  // namespace
 
-std::string plan_adjoint(int64_t n_steps, int every, int64_t buffers, AdjSchedule &out) {
+std::string plan_adjoint(int64_t n_steps, int every, int64_t buffers, int rec_factor,
+                         AdjSchedule &out) {
     out = AdjSchedule();
     if (n_steps < 1) return "n_steps must be >= 1";
     if (every < 1) return "checkpoint_every must be >= 1";
+    if (rec_factor < 1 || rec_factor > 2) return "rec_factor must be 1 or 2";
     const int64_t k = std::min<int64_t>(every, n_steps);
     const int64_t U64 = (n_steps + k - 1) / k;
     if (U64 > 1 << 20) return "too many segments: raise checkpoint_every";
     const int U = (int)U64;
-    const int64_t rb = k + 1;   // buffers of one record range
+    const int64_t rb = rec_factor * (k + 1);   // buffers of one record range
+    out.rec_factor = rec_factor;
     out.segments = U;
     out.seg_steps = (int32_t)k;
     auto ranges_of = [&](int T, int H) -> int {

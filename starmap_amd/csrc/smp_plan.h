@@ -119,6 +119,66 @@ struct Plan {
 std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slot_capacity,
                        int group_size, int max_load_elems, int max_loads_per_group, Plan &out);
 
+// ---- discrete adjoint: the LDS-staged update of one launch ---------------
+// The adjoint of a launch with targets of one parity P and sources of the
+// other Q updates, per target j and point, lambda_j (first adding the
+// transposed flux of mu of the launch after it, whose targets are Q) and
+// mu_j, and accumulates dJ/dsigma from the forward flux W_j.  Both fluxes
+// are sums of coef * (hi - lo) over LDS-staged tiles with the forward
+// geometry: W over the state tiles of the sources, the transposed flux over
+// their mu tiles with coefficient -c_ij, staged so that mu is zero outside
+// the grid and, for a coupling along an extrapolation axis e on which the
+// target is cell-centred, zero on the boundary faces of e (a separate tile
+// of the same mu per such axis; its forward flux there is identically zero).
+// The cell threads form ADJ_PARTS parts, each updating ADJ_TPH of a group's
+// up to ADJ_PARTS*ADJ_TPH targets (consecutive in the order of grid types);
+// each part sums a point's gradient over its targets of a grid type, and
+// the parts' sums are added in a fixed order.
+constexpr int ADJ_TERMS = 4;          // terms per kernel axis and list
+constexpr int ADJ_TPH = 3;            // targets per part of the cell threads and group
+constexpr int ADJ_PARTS = 3;          // parts of the cell threads (256 threads each)
+constexpr int ADJ_RECS = ADJ_PARTS * ADJ_TPH;   // records per group
+constexpr int ADJ_MAXL = 64;          // source tiles staged per group
+constexpr int ADJ_BLOB_LD_OFF = 16;   // load records after the group header
+// load records (DevLoad.pad[0]): 0 state tile, 1 mu tile, 2 + e mu tile
+// with the boundary faces of kernel axis e zero
+
+struct alignas(16) AdjRec {
+    int32_t comp;
+    int32_t gq;         // the target's grid type: index among the launch's
+    int32_t flush;      // the part's last target of its grid type: store the part's sums
+    int32_t jpos;       // comp > 0: dJ/dsigma_s0 accumulates too
+    int32_t sig_off, E_off, dE_off, Q_off;   // staged field: element offset of its tile; -1 scalar /
+                                             // no source; -2-k: field k of the global table
+    int32_t ext[3];
+    int32_t pad0;
+    int64_t uoff;       // byte offset of the component in a state-layout buffer
+    int64_t pad1;
+    double sig_s, E_s, dE_s, pad2;
+    double F_s[2];      // filter factors of slots k=1,2 (1 without a filter)
+    // ADJ_TERMS terms per kernel axis: the forward flux W (state tiles),
+    // then the transposed flux (mu tiles); the low sample's offset in the
+    // tile (element offset; bytes from the start of LDS on the device) and
+    // the coefficient; the high sample lies one stride of the term's axis
+    // further
+    int32_t off[2 * MAXT];
+    double coef[2 * MAXT];
+};
+
+struct AdjPlan {
+    std::vector<DevGroup> groups; // t0 = index of the group's first record (ADJ_RECS per group)
+    std::vector<DevLoad> loads;
+    std::vector<AdjRec> targets;  // ADJ_RECS per group, comp = -1: no target
+    std::vector<int32_t> gcode;   // grid-type codes of the targets, in order
+    int n_slots = 0;
+};
+
+// The schedule of the adjoint update for targets odd (kind 0) or even (1).
+// Fails (the runtime then uses the gather kernels) if a component couples to
+// more than ADJ_TERMS components along an axis or a group does not fit.
+std::string build_adj_plan(const Geometry &g, const TileShape &ts, int kind, int slot_capacity,
+                           AdjPlan &out);
+
 // ---- discrete adjoint: checkpoints, recorded segments, recomputation -----
 // The adjoint steps back over segments of `every` steps.  Reversing a
 // segment needs the forward's launch outputs ("records") of that segment;
@@ -159,14 +219,18 @@ struct AdjSchedule {
     int32_t tail = 0;           // T
     int32_t slots = 0;          // c
     int32_t work = 0;           // 1 if the schedule uses the scratch state
-    int32_t ranges = 0;         // record ranges of seg_steps+1 buffers each
+    int32_t ranges = 0;         // record ranges of rec_factor*(seg_steps+1) buffers each
+    int32_t rec_factor = 1;
     int64_t recomputed = 0;     // head segments recomputed into records
     int64_t extra_advances = 0; // further segment advances from checkpoints
     int64_t buffers = 0;        // state-sized buffers at the peak
     std::vector<AdjAction> fwd, rev;
 };
 
-// buffers < 0: no budget (a checkpoint per segment, no tail)
-std::string plan_adjoint(int64_t n_steps, int every, int64_t buffers, AdjSchedule &out);
+// buffers < 0: no budget (a checkpoint per segment, no tail); rec_factor:
+// state-sized buffers per record of two launches (1: the launches' states; 2:
+// and the W+Q they stored)
+std::string plan_adjoint(int64_t n_steps, int every, int64_t buffers, int rec_factor,
+                         AdjSchedule &out);
 
 }  // namespace smp

@@ -16,8 +16,8 @@ from starmap_amd.adjoint import adjoint_schedule
 WORK, TAIL_START, PREV, NONE = -2, -3, -4, -1
 
 
-def replay(n_steps, every, buffers):
-    st, fwd, rev = adjoint_schedule(n_steps, every, buffers)
+def replay(n_steps, every, buffers, rec_factor=1):
+    st, fwd, rev = adjoint_schedule(n_steps, every, buffers, rec_factor)
     k = min(every, n_steps)
     U = -(-n_steps // k)
     assert st["segments"] == U and st["steps_per_segment"] == k
@@ -86,7 +86,8 @@ def replay(n_steps, every, buffers):
     assert nxt == -1, "every segment is reversed"
     assert recomputed == st["recomputed_segments"] == H
     assert advanced == st["extra_segment_advances"]
-    peak = (max(used_ranges) + 1 if used_ranges else 0) * (k + 1) + len(used_slots) + work + tail_start
+    peak = ((max(used_ranges) + 1 if used_ranges else 0) * rec_factor * (k + 1) + len(used_slots) + work
+            + tail_start)
     assert peak <= st["peak_buffers"]
     if buffers >= 0:
         assert st["peak_buffers"] <= buffers
@@ -106,24 +107,28 @@ def test_default_schedule_is_one_checkpoint_per_segment(n_steps, every):
     assert st["peak_buffers"] == U + 2 * (k + 1)
 
 
+@pytest.mark.parametrize("rec_factor", [1, 2])
 @pytest.mark.parametrize("n_steps,every", CASES)
-def test_budget_schedules_replay_and_trade_memory_for_recomputation(n_steps, every):
+def test_budget_schedules_replay_and_trade_memory_for_recomputation(n_steps, every, rec_factor):
+    """rec_factor 2: a step's records hold the launches' W+Q beside their
+    states, so a recorded segment takes twice the buffers."""
     k = min(every, n_steps)
     U = -(-n_steps // k)
-    full = U * (k + 1) + 1
-    budgets = sorted({full, full + 5, full - 1, full // 2, full // 4, 2 * (k + 1) + 4,
-                      2 * (k + 1) + 3, 3 * (k + 1) + 2, U + 2 * (k + 1) + 1})
+    full = U * rec_factor * (k + 1) + 1
+    rb = rec_factor * (k + 1)
+    budgets = sorted({full, full + 5, full - 1, full // 2, full // 4, 2 * rb + 4,
+                      2 * rb + 3, 3 * rb + 2, U + 2 * rb + 1})
     cost = []
     for b in budgets:
-        if b < 2 * (k + 1) + 3:
+        if b < 2 * rec_factor * (k + 1) + 3:
             continue
-        st = replay(n_steps, every, b)
+        st = replay(n_steps, every, b, rec_factor)
         cost.append((b, st["recomputed_segments"] + st["extra_segment_advances"]))
     # more memory never costs more forward work
     for (b0, c0), (b1, c1) in zip(cost, cost[1:]):
         assert c1 <= c0, (b0, c0, b1, c1)
     # everything recorded in the forward pass when it fits
-    st = replay(n_steps, every, full)
+    st = replay(n_steps, every, full, rec_factor)
     assert st["recorded_in_forward"] == U and st["recomputed_segments"] == 0
 
 
