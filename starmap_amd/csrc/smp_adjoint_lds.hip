@@ -83,7 +83,7 @@ size_t adj_lds_overhead(int ncf, int ngt, int elem) {
     return 3 * (size_t)BLOB_SZ + (size_t)ncf * AHALF * elem + 2 * AP * (size_t)ngt * AHALF * sizeof(double);
 }
 
-template <typename T, int NDIM, bool FUSE, bool GEN>
+template <typename T, int NDIM, bool FUSE, bool GEN, bool HASW>
 __global__ __launch_bounds__(ADJ_THREADS) void adj_lds_kernel(const AdjTileArgs<T> a) {
     constexpr int NT = ADJ_TERMS;
     constexpr int TL0 = adj_tile_dim(NDIM, 0) + 1;   // LDS slot extents
@@ -288,12 +288,14 @@ __global__ __launch_bounds__(ADJ_THREADS) void adj_lds_kernel(const AdjTileArgs<
     auto rec_of = [&](int gq) {
         return (const SMP_CONST AdjRec *)(a.recs) + (int64_t)gq * ADJ_RECS + half * ADJ_TPH;
     };
-    auto load_targets = [&](int gq, T *u0, T *lm) {
+    SMP_GLOBAL const char *w_ld = HASW ? glb(reinterpret_cast<const char *>(a.W)) + pix : nullptr;
+    auto load_targets = [&](int gq, T *u0, T *lm, T *wv) {
         const auto r = rec_of(gq);
 #pragma unroll
         for (int i = 0; i < ADJ_TPH; ++i) {
             u0[i] = *reinterpret_cast<SMP_GLOBAL const T *>(u0_ld + r[i].uoff);
             lm[i] = *reinterpret_cast<SMP_GLOBAL const T *>(lam_ld + r[i].uoff);
+            if (HASW) wv[i] = *reinterpret_cast<SMP_GLOBAL const T *>(w_ld + r[i].uoff);
         }
     };
     const int lb = lbase * (int)sizeof(T);
@@ -330,18 +332,18 @@ __global__ __launch_bounds__(ADJ_THREADS) void adj_lds_kernel(const AdjTileArgs<
         return v;
     };
     double ga = 0, gs = 0;
-    auto update = [&](int gq, const T *u0, const T *lm) {
+    auto update = [&](int gq, const T *u0, const T *lm, const T *wv) {
         const auto rb = rec_of(gq);
 #pragma unroll
         for (int i = 0; i < ADJ_TPH; ++i) {
             const auto &r = rb[i];
             const bool act = has_cell & (g0 < r.ext[0]) & (g1 < r.ext[1]) & (g2 < r.ext[2]);
-            const double w = flux(r.off, r.coef);
+            const double w = HASW ? 0.0 : flux(r.off, r.coef);
             double lam = (double)lm[i];
             if (FUSE) lam = (double)(T)(lam + flux(r.off + MAXT, r.coef + MAXT));
             const double sg = coef(r.sig_off, r.sig_s);
             const double E = coef(r.E_off, r.E_s);
-            const double wq = w + coef(r.Q_off, 0.0);
+            const double wq = HASW ? (double)wv[i] : w + coef(r.Q_off, 0.0);
             const double dE = coef(r.dE_off, r.dE_s);
             double u[2];
             u[0] = (double)u0[i];
@@ -392,16 +394,16 @@ __global__ __launch_bounds__(ADJ_THREADS) void adj_lds_kernel(const AdjTileArgs<
         gacc[(2 * AP * q + 2 * half + 1) * AHALF + ctid] = 0.0;
     }
     __syncthreads();
-    T u0A[ADJ_TPH], lmA[ADJ_TPH], u0B[ADJ_TPH], lmB[ADJ_TPH];
-    if (ng > 0) load_targets(0, u0A, lmA);
+    T u0A[ADJ_TPH], lmA[ADJ_TPH], wA[ADJ_TPH], u0B[ADJ_TPH], lmB[ADJ_TPH], wB[ADJ_TPH];
+    if (ng > 0) load_targets(0, u0A, lmA, wA);
     __syncthreads();
     for (int gq = 0; gq < ng; gq += 2) {
-        update(gq, u0A, lmA);
-        if (gq + 1 < ng) load_targets(gq + 1, u0B, lmB);
+        update(gq, u0A, lmA, wA);
+        if (gq + 1 < ng) load_targets(gq + 1, u0B, lmB, wB);
         __syncthreads();
         if (gq + 1 >= ng) break;
-        update(gq + 1, u0B, lmB);
-        if (gq + 2 < ng) load_targets(gq + 2, u0A, lmA);
+        update(gq + 1, u0B, lmB, wB);
+        if (gq + 2 < ng) load_targets(gq + 2, u0A, lmA, wA);
         __syncthreads();
     }
     __syncthreads();
@@ -422,28 +424,36 @@ __global__ __launch_bounds__(ADJ_THREADS) void adj_lds_kernel(const AdjTileArgs<
     }
 }
 
-template <typename T, int NDIM, bool FUSE, bool GEN>
+template <typename T, int NDIM, bool FUSE, bool GEN, bool HASW>
 static hipError_t launch_adj_v(const AdjTileArgs<T> &a, size_t lds_bytes, hipStream_t s) {
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void *)adj_lds_kernel<T, NDIM, FUSE, GEN>,
+        hipError_t e = hipFuncSetAttribute((const void *)adj_lds_kernel<T, NDIM, FUSE, GEN, HASW>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
     const int nb = ((a.n_tiles + 7) / 8) * 8;
-    hipLaunchKernelGGL((adj_lds_kernel<T, NDIM, FUSE, GEN>), dim3(nb), dim3(ADJ_THREADS), lds_bytes, s, a);
+    hipLaunchKernelGGL((adj_lds_kernel<T, NDIM, FUSE, GEN, HASW>), dim3(nb), dim3(ADJ_THREADS), lds_bytes, s,
+                       a);
     return hipGetLastError();
+}
+
+template <typename T, int NDIM, bool HASW>
+static hipError_t launch_adj_w(const AdjTileArgs<T> &a, bool fuse, bool gen, size_t lds_bytes,
+                               hipStream_t s) {
+    if (fuse)
+        return gen ? launch_adj_v<T, NDIM, true, true, HASW>(a, lds_bytes, s)
+                   : launch_adj_v<T, NDIM, true, false, HASW>(a, lds_bytes, s);
+    return gen ? launch_adj_v<T, NDIM, false, true, HASW>(a, lds_bytes, s)
+               : launch_adj_v<T, NDIM, false, false, HASW>(a, lds_bytes, s);
 }
 
 template <typename T, int NDIM>
 static hipError_t launch_adj_d(const AdjTileArgs<T> &a, bool fuse, bool gen, size_t lds_bytes,
                                hipStream_t s) {
-    if (fuse)
-        return gen ? launch_adj_v<T, NDIM, true, true>(a, lds_bytes, s)
-                   : launch_adj_v<T, NDIM, true, false>(a, lds_bytes, s);
-    return gen ? launch_adj_v<T, NDIM, false, true>(a, lds_bytes, s)
-               : launch_adj_v<T, NDIM, false, false>(a, lds_bytes, s);
+    return a.W ? launch_adj_w<T, NDIM, true>(a, fuse, gen, lds_bytes, s)
+               : launch_adj_w<T, NDIM, false>(a, fuse, gen, lds_bytes, s);
 }
 
 template <typename T>

@@ -146,6 +146,7 @@ struct smp_problem {
     void *adj_lds_cf[2] = {nullptr, nullptr}, *adj_lds_gf[2] = {nullptr, nullptr};
     void *adj_zeros = nullptr;   // 64 zero dwords: boundary faces of masked mu tiles
     int adj_lds_ncf[2] = {0, 0};
+    bool adj_recw = false;   // the recording launches store W+Q (set per smp_adjoint call)
     bool adj_lds_gen[2] = {false, false};   // some coefficient field is read from global memory
     size_t adj_lds_bytes[2] = {0, 0};
     std::map<const void *, void *> adj_dE;   // dE/dsigma field of each sigma field
@@ -876,8 +877,22 @@ int refresh_coefs(smp_problem *h) {
             }
         }
         h->gen[kind] = false;
-        for (const smp::TgtRec &r : recs)
-            if (r.sig_off == -2 || r.E_off == -2 || r.Q_off == -2 || r.F[0] || r.F[1]) h->gen[kind] = true;
+        for (smp::TgtRec &r : recs) {
+            const bool glob = r.sig_off == -2 || r.E_off == -2 || r.Q_off == -2 || r.F[0] || r.F[1];
+            if (glob) {
+                h->gen[kind] = true;
+            } else if (h->elem == 4) {
+                // the scalar factors of the FP32 half-step: dt/2 sigma E,
+                // dt/2 E, 1-F of the filter slots, sigma
+                const double hd = h->dt_decay / 2;
+                r.f32[0] = (float)(hd * r.sig_s * r.E_s);
+                r.f32[1] = (float)(hd * r.E_s);
+                r.f32[2] = (float)(1.0 - r.F_s[0]);
+                r.f32[3] = (float)(1.0 - r.F_s[1]);
+                r.f32[4] = (float)r.sig_s;
+                r.f32[5] = 0.0f;
+            }
+        }
         if (!h->cf_dev[kind]) {
             int rc = h->dalloc(&h->cf_dev[kind], sizeof(void *) * smp::MAX_CFIELDS);
             if (rc) return rc;
@@ -1120,7 +1135,8 @@ int adjoint_lds_setup(smp_problem *h, const std::vector<smp::AdjComp<T>> &comp) 
             const size_t per_block = (160 * 1024 - 256) / blocks;
             const size_t fixed = smp::adj_lds_overhead((int)table.size(), ngt_of, h->elem);
             if (per_block <= fixed + 2 * slot_bytes) continue;
-            err = smp::build_adj_plan(g, ts, kind, (int)((per_block - fixed) / slot_bytes), plan);
+            err = smp::build_adj_plan(g, ts, kind, (int)((per_block - fixed) / slot_bytes), h->adj_recw,
+                                      plan);
             if (err.empty()) break;
         }
         if (!err.empty()) return unavailable(err);
@@ -1392,20 +1408,18 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
         rc = refresh_coefs<T>(h);
         if (rc) return rc;
     }
+    // Recording launches also store W+Q of their targets, so that the update
+    // needs no forward flux of its own (the gather update: 16 fewer gathers
+    // per target in 2D, about 25% less time per launch; the LDS update: no
+    // state tiles), for twice the memory per recorded segment (under a
+    // budget, fewer segments recorded in the forward pass).  Not with
+    // global-memory coefficients or with SMP_ADJ_RECORD_W=0.
+    const char *rw_env = std::getenv("SMP_ADJ_RECORD_W");
+    const bool recW = !h->gen[0] && !h->gen[1] && !(rw_env && std::string(rw_env) == "0");
+    h->adj_recw = recW;
     rc = adjoint_tables<T>(h, dt);
     if (rc) return rc;
     const size_t st = (size_t)h->slab * h->elem * h->geo.n_sys;
-    // Recording launches also store W+Q of their targets, so that the
-    // gather update needs no forward flux of its own: 16 fewer gathers per
-    // target in 2D, about 25% less time per update launch, for twice the
-    // memory per recorded segment (under a budget, fewer segments recorded
-    // in the forward pass).  Not with global-memory coefficients, with the
-    // LDS update (which stages the sources anyway) or with
-    // SMP_ADJ_RECORD_W=0.
-    const char *rw_env = std::getenv("SMP_ADJ_RECORD_W");
-    const char *kn_env = std::getenv("SMP_ADJ_KERNEL");
-    const bool recW = !h->gen[0] && !h->gen[1] && !(rw_env && std::string(rw_env) == "0") &&
-                      !(kn_env && std::string(kn_env) == "lds");
     smp::AdjSchedule sc;
     {
         const int64_t nbuf = h->adj_budget > 0 ? h->adj_budget / (int64_t)st : -1;
@@ -1572,13 +1586,14 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
         a.ndim = h->geo.ndim;
         return a;
     };
-    auto lds_args = [&](const void *U, const void *U0, int kind, int napp, int f0, int f1) {
+    auto lds_args = [&](const void *U, const void *U0, const void *W, int kind, int napp, int f0, int f1) {
         smp::AdjTileArgs<T> a;
         std::memset(&a, 0, sizeof(a));
         const smp::TileShape &ts = h->adj_ts;
         const smp::AdjPlan &p = h->adj_plan[kind];
         a.U = (const T *)U;
         a.U0 = (const T *)U0;
+        a.W = (const T *)W;
         a.lam = (T *)h->lam;
         a.mu = (T *)h->mu;
         a.Ga = (double *)h->G;
@@ -1638,7 +1653,7 @@ int run_adjoint(smp_problem *h, double dt, int64_t n_steps, int every) {
             const int32_t *tl = kind ? list_even : list_odd, *sl = kind ? list_odd : list_even;
             const int ntl = kind ? nl_even : nl_odd, nsl = kind ? nl_odd : nl_even;
             if (h->adj_lds_ok)
-                HIP_TRY(smp::launch_adj_lds<T>(lds_args(U, U0, kind, f.napp, f.fslot[0], f.fslot[1]),
+                HIP_TRY(smp::launch_adj_lds<T>(lds_args(U, U0, wrec(r, L), kind, f.napp, f.fslot[0], f.fslot[1]),
                                                h->geo.ndim, L + 1 < nl, h->adj_lds_gen[kind],
                                                h->adj_lds_bytes[kind], h->stream));
             else

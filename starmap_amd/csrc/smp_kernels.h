@@ -257,6 +257,7 @@ template <typename T>
 struct AdjTileArgs {
     const T *U;               // the launch's sources (the state after launch L-1)
     const T *U0;              // its targets before it (after launch L-2)
+    const T *W;               // W+Q its targets had, as the forward launch stored it (or null)
     T *lam, *mu;              // adjoint state and adjoint of the flux (state layout)
     double *Ga, *Gs;          // dJ/dsigma_a, dJ/dsigma_s0: 8 grid-type slabs each
     T *sink;                  // 1024 elements written by lanes without a target point
@@ -294,7 +295,8 @@ constexpr int adj_tile_dim(int ndim, int e) {
 
 // ndim: active kernel axes; fuse: add the transposed flux of the launch
 // after first (mu tiles staged); gen: some coefficient is a field read from
-// global memory (more fields than the LDS stages)
+// global memory (more fields than the LDS stages); a.W not null: W+Q from
+// the forward launch's record (the plan then stages no state tiles)
 template <typename T>
 hipError_t launch_adj_lds(const AdjTileArgs<T> &a, int ndim, bool fuse, bool gen, size_t lds_bytes,
                           hipStream_t s);

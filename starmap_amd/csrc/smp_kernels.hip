@@ -108,6 +108,7 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
                  << 32) |
                     (uint32_t)__builtin_amdgcn_readfirstlane(
                         (int)(uint32_t)__builtin_bit_cast(uint64_t, (double)a.half_dt)));
+    const float half_dt_f = (float)half_dt;
     const int fslot0 = __builtin_amdgcn_readfirstlane(a.fslot[0]);
     const int fslot1 = __builtin_amdgcn_readfirstlane(a.fslot[1]);
     const int lane = tid & 63;
@@ -312,14 +313,20 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     // offsets are byte offsets from the start of LDS
     const int lb = lbase * (int)sizeof(T);
     // The flux sum W computes in the state precision (FP32 terms with FP32
-    // coefficients for the FP32 state); the half-step update and the filter
-    // factor in FP64 for both states.  Scalar coefficients -- the decay
-    // rate, its factor E, dt/2 and the filter factors -- are then used
-    // unrounded: rounded to FP32 once and applied every half-step, each would
-    // bias the FP32 state by the same relative amount per application, an
+    // coefficients for the FP32 state).  The half-step of the FP64 state,
+    // and of the FP32 state with fields read from global memory (GEN), is
+    // computed in FP64.  The FP32 state otherwise computes it in FP32 as a
+    // change added to u once,
+    //     u <- u + (c - (1-F) u - (1-F) c),  c = (dt/2 E)(W+Q) - (dt/2 s E) u,
+    // so that every factor near 1 enters through its small part x = 1-F or
+    // dt/2 s E: a factor rounded to FP32 once and applied every half-step
+    // biases the state by the same relative amount at each application, an
     // error that grows linearly with the step count (about 1e-5 after 500
-    // filtered steps).  With packed FP32 instructions off, the FP64 update
-    // costs no more issue slots than the FP32 one.
+    // filtered steps if F itself is rounded), where the rounded x leaves a
+    // bias of x times that.  Applying decay, source and filter one after the
+    // other, each rounded, lost the changes below half an ulp of u at each of
+    // three roundings: 6.5 times the FP64-update error after the 915 steps of
+    // the 2D P39 lattice.
     typedef T A;
     typedef double D;
     // coefficient: staged field (off >= 0), scalar (off == -1) or, in the
@@ -359,6 +366,45 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
 #pragma unroll
             for (int k = 0; k < PC; ++k) {
                 const bool ak = k ? act1 : act;
+                if constexpr (sizeof(T) == 4 && !GEN) {
+                    // every record value is read before the selects: a read
+                    // in one arm only would become a branch around a scalar
+                    // load, whose wait also drains the LDS reads in flight
+                    const float eps_s = r.f32[0], B_s = r.f32[1], dF0 = r.f32[2], dF1 = r.f32[3],
+                                sig_s = r.f32[4];
+                    const float sgf = (float)cfa[(r.sig_off > 0 ? r.sig_off : 0) + PC * ctid + k];
+                    const float Ef = (float)cfa[(r.E_off > 0 ? r.E_off : 0) + PC * ctid + k];
+                    const float Qf = (float)cfa[(r.Q_off > 0 ? r.Q_off : 0) + PC * ctid + k];
+                    const float Bf = half_dt_f * Ef;
+                    const float B = r.E_off >= 0 ? Bf : B_s;
+                    // sigma times dt/2 E: both scalars (the record's product),
+                    // or a field times a scalar or a field
+                    const float eps = r.sig_off >= 0 ? B * sgf : (r.E_off >= 0 ? Bf * sig_s : eps_s);
+                    const float wq = (float)w[k] + (r.Q_off >= 0 ? Qf : 0.0f);
+                    if (WOUT) {
+                        SMP_GLOBAL T *dw = ak ? reinterpret_cast<SMP_GLOBAL T *>(wo_st + r.uoff + k * sizeof(T))
+                                              : sink + PC * tid + k;
+                        *dw = (T)wq;
+                    }
+                    float u = (float)u_in[PC * i + k];
+#pragma unroll
+                    for (int app = 0; app < NAPP; ++app) {
+                        // (u + c)(1 - dF) = u + (c - dF u - dF c), with the
+                        // change c = dt/2 E (W+Q) - dt/2 s E u: the change is
+                        // summed in FP32 relative to its own size and added
+                        // to u once, one rounding per half-step
+                        const bool s1 = (app == 0 ? fslot0 : fslot1) != 0;
+                        const float dF = s1 ? dF1 : dF0;   // 0 without a filter
+                        const float c = __builtin_fmaf(B, wq, -(eps * u));
+                        float d = __builtin_fmaf(-dF, u, c);
+                        d = __builtin_fmaf(-dF, c, d);
+                        u = u + d;
+                    }
+                    SMP_GLOBAL T *dst = ak ? reinterpret_cast<SMP_GLOBAL T *>(ug_st + r.uoff + k * sizeof(T))
+                                           : sink + PC * tid + k;
+                    *dst = (T)u;
+                    continue;
+                }
                 const D sig = coef(r.sig_off, r.sig_p, r.sig_s, k);
                 const D E = coef(r.E_off, r.E_p, r.E_s, k);
                 const D wq = (D)w[k] + coef(r.Q_off, r.Q_p, 0.0, k);

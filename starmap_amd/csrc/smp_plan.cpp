@@ -329,7 +329,7 @@ std::string build_plan(const Geometry &g, const TileShape &ts, int kind, int slo
 }
 
 std::string build_adj_plan(const Geometry &g, const TileShape &ts, int kind, int slot_capacity,
-                           AdjPlan &out) {
+                           bool with_w, AdjPlan &out) {
     out = AdjPlan();
     if (ts.pair) return "the adjoint update takes one cell per thread";
     if (ADJ_TERMS * g.ndim > MAXT) return "too many kernel axes";
@@ -374,7 +374,8 @@ std::string build_adj_plan(const Geometry &g, const TileShape &ts, int kind, int
             if (e < 0) return "coupling along an axis without cells";
             if (M.rp[j + 1] - M.rp[j] > ADJ_TERMS || (int)cols[d][j].size() > ADJ_TERMS)
                 return "more than 4 couplings of a component along an axis";
-            for (int p = M.rp[j]; p < M.rp[j + 1]; ++p) s.insert(M.ci[p]);
+            if (!with_w)
+                for (int p = M.rp[j]; p < M.rp[j + 1]; ++p) s.insert(M.ci[p]);
             for (const auto &ic : cols[d][j]) s.insert(mu_id(j, ic.first, e));
         }
         src[t].assign(s.begin(), s.end());
@@ -415,7 +416,7 @@ std::string build_adj_plan(const Geometry &g, const TileShape &ts, int kind, int
                     const Csr &M = g.M[d];
                     if (M.rp.empty()) continue;
                     int nf = 0, nb = 0;
-                    for (int pp = M.rp[j]; pp < M.rp[j + 1]; ++pp, ++nf) {
+                    for (int pp = M.rp[j]; pp < M.rp[j + 1] && !with_w; ++pp, ++nf) {
                         const int k = M.ci[pp];
                         r.off[ADJ_TERMS * e + nf] = term_offset(g, ts, q.slot(k), &g.gt[3 * k], e);
                         r.coef[ADJ_TERMS * e + nf] = -M.cv[pp] / g.h[d];

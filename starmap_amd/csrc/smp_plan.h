@@ -53,7 +53,15 @@ struct alignas(16) TgtRec {
     int32_t sig_off, E_off;
     int32_t Q_off;
     int32_t ext[3];     // target extent per kernel axis
-    const void *sig_p, *E_p, *Q_p;
+    union {
+        struct {
+            const void *sig_p, *E_p, *Q_p;   // fields read from global memory (off -2)
+        };
+        // a record with no field read from global memory, FP32-state device
+        // copy: dt/2*sigma*E and dt/2*E of scalar coefficients, 1-F of the
+        // filter slots k=1,2, sigma (the FP32 half-step, smp_kernels.hip)
+        float f32[6];
+    };
     int64_t uoff;       // byte offset of the component in the state (device copy)
     const void *F[2];   // filter fields of slots k=1,2 (or null: F_s)
     double sig_s, E_s;
@@ -176,8 +184,10 @@ struct AdjPlan {
 // The schedule of the adjoint update for targets odd (kind 0) or even (1).
 // Fails (the runtime then uses the gather kernels) if a component couples to
 // more than ADJ_TERMS components along an axis or a group does not fit.
+// with_w: the forward launches stored W+Q, so neither the state tiles nor
+// the forward-flux terms are needed
 std::string build_adj_plan(const Geometry &g, const TileShape &ts, int kind, int slot_capacity,
-                           AdjPlan &out);
+                           bool with_w, AdjPlan &out);
 
 // ---- discrete adjoint: checkpoints, recorded segments, recomputation -----
 // The adjoint steps back over segments of `every` steps.  Reversing a

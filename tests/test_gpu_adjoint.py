@@ -363,9 +363,10 @@ def _adjoint_results(prob, precision, every, kernel, monkeypatch, seed_fn=None, 
     return lam, grads
 
 
+@pytest.mark.parametrize("record_w", ["0", "1"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("case", ["2d_extrapolation_filter", "2d_periodic", "3d_mixed"])
-def test_lds_adjoint_update_equals_gather_kernel(case, precision, monkeypatch):
+def test_lds_adjoint_update_equals_gather_kernel(case, precision, record_w, monkeypatch):
     """The LDS-staged adjoint update (state and mu tiles in LDS, the
     transposed flux as forward-form differences of mu extended by zero,
     boundary faces masked per coupling axis) against the gather kernels,
@@ -375,7 +376,9 @@ def test_lds_adjoint_update_equals_gather_kernel(case, precision, monkeypatch):
     (the LDS kernel adds two partial sums per grid type and point).  Covers
     extrapolation and periodic axes, the substep filter, sources and
     spatially varying materials, 2D and 3D.  SMP_ADJ_KERNEL=lds makes the
-    library fail rather than fall back to the gather kernels."""
+    library fail rather than fall back to the gather kernels.  With
+    record_w = 1 both read W+Q from the forward launches' records (the LDS
+    kernel then stages no state tiles)."""
     if case == "3d_mixed" and precision == "fp64":
         pytest.skip("3D FP64: two groups' source tiles (480-element FP64 slots) do not fit in the "
                     "LDS; the library runs the gather kernels for it")
@@ -385,8 +388,8 @@ def test_lds_adjoint_update_equals_gather_kernel(case, precision, monkeypatch):
         prob, _, _ = small(n=18, n_mom=7, steps=6, bc=(0, 0))
     else:
         prob = small_3d(n=(7, 6, 5), n_mom=3, steps=4)
-    ref = _adjoint_results(prob, precision, 3, "gather", monkeypatch, record_w="0")
-    new = _adjoint_results(prob, precision, 3, "lds", monkeypatch, record_w="0")
+    ref = _adjoint_results(prob, precision, 3, "gather", monkeypatch, record_w=record_w)
+    new = _adjoint_results(prob, precision, 3, "lds", monkeypatch, record_w=record_w)
     for a, b in zip(ref[0], new[0]):
         assert np.array_equal(a, b), (case, precision, float(np.abs(a - b).max()))
     for a, b in zip(ref[1], new[1]):
