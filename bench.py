@@ -55,12 +55,8 @@ def b_alg_per_mcu(n_even, n_odd, n_sys, s):
 
 
 def sync():
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
+    import torch
+    torch.cuda.synchronize()
 
 
 TUNE = True

@@ -69,6 +69,11 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError("libstarmap_hip.so is not built (run `python -m starmap_amd.build`)")
+    # torch ships its own HIP runtime (libamdhip64.so.7).  Loaded first, the
+    # library binds the system runtime under the same soname, and torch then
+    # finds no GPU in the process; with torch loaded first both share its
+    # runtime, so torch's streams and synchronisation see the library's work.
+    import torch  # noqa: F401
     L = ctypes.CDLL(LIB_PATH)
     vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
     sig = {
