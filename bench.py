@@ -294,16 +294,17 @@ def adjoint_leg(n=512, n_mom=15, steps=2000, every=50, fd_cells=64, fd_cells_s=1
         ga, gs = s.cell_gradients(owner, n * n)
         same = bool(np.array_equal(ga, ga0) and np.array_equal(gs, gs0))
         # algorithmic bytes of one fused update launch (targets of one
-        # parity, mean over the two): per target and point U0, lambda
-        # read and written, mu written; per source and point its state and
-        # its mu read once; per grid type of the targets and point the two
-        # FP64 gradients read and written (coefficient fields not counted)
+        # parity, mean over the two): per target and point U0 and the
+        # recorded W+Q read, lambda read and written, mu written; per source
+        # and point its mu read once; per grid type of the targets and point
+        # the two FP64 gradients read and written (coefficient fields not
+        # counted)
         par0 = sum(s.gkey[0])
         parity = [(sum(g) - par0) % 2 for g in s.gkey]
         n_p = [parity.count(0), parity.count(1)]
         n_g = [len({g for g, q in zip(s.gkey, parity) if q == k}) for k in (0, 1)]
         e = 4
-        upd_bytes = float(np.mean([n * n * (4 * e * n_p[k] + 2 * e * n_p[1 - k] + 32 * n_g[k])
+        upd_bytes = float(np.mean([n * n * (5 * e * n_p[k] + e * n_p[1 - k] + 32 * n_g[k])
                                    for k in (0, 1)]))
     finally:
         s.close()

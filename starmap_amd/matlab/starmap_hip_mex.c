@@ -32,7 +32,12 @@
  *   starmap_hip_mex('set_schedule', h, tile, blocks_per_cu, cell_waves)
  *   starmap_hip_mex('adjoint_reset', h)
  *   starmap_hip_mex('adjoint_seed', h, j, dJdU)
+ *   starmap_hip_mex('adjoint_memory', h, bytes)      device-memory budget (0: none)
  *   starmap_hip_mex('adjoint', h, dt, n_steps, checkpoint_every)
+ *   s = starmap_hip_mex('adjoint_stats', h)          1 x 8: segments, steps per segment,
+ *                                                    recorded in the forward pass, checkpoint
+ *                                                    slots, recomputed, further advances,
+ *                                                    buffers and bytes at the peak
  *   G = starmap_hip_mex('adjoint_get', h, what, j)   what 0 lambda, 1 dJ/dsigma_a,
  *                                                    2 dJ/dsigma_s0
  */
@@ -382,6 +387,14 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
     } else if (!strcmp(cmd, "adjoint")) {
         check(smp_adjoint(h, scalar_arg(prhs[2], "dt"), (int64_t)scalar_arg(prhs[3], "n_steps"),
                           (int32_t)scalar_arg(prhs[4], "checkpoint_every")));
+    } else if (!strcmp(cmd, "adjoint_memory")) {
+        if (nrhs != 3) usage("adjoint_memory: h, bytes");
+        check(smp_adjoint_set_memory(h, (int64_t)scalar_arg(prhs[2], "bytes")));
+    } else if (!strcmp(cmd, "adjoint_stats")) {
+        int64_t st[8];
+        check(smp_adjoint_stats(h, st));
+        plhs[0] = mxCreateDoubleMatrix(1, 8, mxREAL);
+        for (int k = 0; k < 8; ++k) mxGetDoubles(plhs[0])[k] = (double)st[k];
     } else if (!strcmp(cmd, "adjoint_get")) {
         const int32_t what = (int32_t)scalar_arg(prhs[2], "what");
         const int32_t j = comp_arg(prhs[3]);
