@@ -490,6 +490,8 @@ def main():
     ap.add_argument("--fd-cells", type=int, default=64,
                     help="cells of the adjoint leg's finite-difference check of dJ/dsigma_a "
                          "(dJ/dsigma_s0 at the first 16 of them)")
+    ap.add_argument("--adjoint-memory-frac", type=float, default=0.7,
+                    help="the adjoint leg's memory budget as a fraction of the free device memory")
     ap.add_argument("--no-tune", action="store_true",
                     help="library default schedules instead of the autotuner's (profiling runs)")
     args = ap.parse_args()
@@ -522,7 +524,8 @@ def main():
         out["batched"] = batched_leg(out.get("value") if out.get("dtype") == "f64" else None,
                                      n_instances=args.batch, check=not args.no_batch_check)
     if "adjoint" in legs:
-        out["adjoint"] = adjoint_leg(steps=args.adjoint_steps, fd_cells=args.fd_cells)
+        out["adjoint"] = adjoint_leg(steps=args.adjoint_steps, fd_cells=args.fd_cells,
+                                     memory_frac=args.adjoint_memory_frac)
     if "cpu" in legs:
         inv, nt, cl = cpu_baseline(c2_steps=args.cpu_steps)
         c2 = cl["c2"]["all_physical_cores"]

@@ -24,6 +24,7 @@ import inspect
 import numpy as np
 import pytest
 
+from oracle import starmap_oracle as so
 from starmap_amd import init as I
 from starmap_amd import problems as P
 from starmap_amd.observables import moment_terms
@@ -108,3 +109,54 @@ def test_gauss3d_p7_200_mass_decay(precision, tol):
         assert abs(m4 - expect) <= tol * expect
     finally:
         s.close()
+
+
+def varied_ic(x, y, z, k):
+    """A different field in every component k (1-based), with wavelengths from
+    about 8 to 160 cells of the full-size grids: deterministic in (point, k),
+    so the oracle and the HIP path start from the same state."""
+    f = (k * 0.6180339887) % 1.0
+    g = (k * 0.4142135624) % 1.0
+    return 0.5 * np.cos((5.0 + 95.0 * f) * x + (3.0 + 60.0 * g) * y + (7.0 + 40.0 * f * g) * z
+                        + 6.283 * g) / (1.0 + 0.01 * k)
+
+
+def varied_ic_2d(x, y, k):
+    return varied_ic(x, y, 0.0, k)
+
+
+@pytest.mark.parametrize("workload", ["c2", "c3"])
+def test_full_size_matches_oracle(workload):
+    """C2 (2D P39 lattice 1000^2: substep filter, source, absorber blocks)
+    and C3 (3D P7 200^3 periodic, anisotropic scattering) at their full sizes
+    against the oracle, every moment field at 1e-12 relative L2 (the measure
+    of the parity tests): a varied state in every component, two steps
+    (final time 1.5 dt, so the output interpolates between the steps' states,
+    starmap_solver.m:425-428)."""
+    from tests.test_gpu_parity import rel_errors
+    if workload == "c2":
+        prob = P.lattice_2d(n=1000, n_mom=39, tfinal=3.2)
+    else:
+        prob = P.gauss_periodic_3d(n=200, n_mom=7, tfinal=0.75)
+    prob["ic"] = varied_ic_2d if workload == "c2" else varied_ic
+    par = I.starmap_init(prob)
+    s = HipSolver(par, device=-1, precision="fp64")   # host only: the first step's dt
+    try:
+        dt = min(s.dt_hyp, par["tfinal"] / par["cn"]) * par["cn"]
+    finally:
+        s.close()
+    prob.update(tfinal=1.5 * dt, t_plot=[1.5 * dt], mom_output="all")
+    print("oracle", workload, flush=True)
+    ref, rpar = so.starmap_solver(so.starmap_init(prob), verbose=False, return_state=True)
+    print("hip", workload, flush=True)
+    s = HipSolver(I.starmap_init(prob), precision="fp64")
+    try:
+        out = s.run(verbose=False)
+        assert s.step_count == rpar["_oracle"]["step_count"] == 2
+        assert s.dt == rpar["_oracle"]["dt"]
+    finally:
+        s.close()
+    assert len(out) == len(ref) == (820 if workload == "c2" else 64)
+    errs = rel_errors(out, ref)
+    print("full size", workload, "worst", errs.max(), flush=True)
+    assert errs.max() <= 1e-12, (workload, errs.max(), int(errs.argmax()))

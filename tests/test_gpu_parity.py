@@ -46,6 +46,9 @@ def rel_errors(out, ref):
 
 
 def run_both(prob, precision="fp64"):
+    """The oracle and the HIP path on the same problem, every moment field in
+    the outputs (whatever the problem's own mom_output)."""
+    prob = dict(prob, mom_output="all")
     ref, rpar = so.starmap_solver(so.starmap_init(prob), verbose=False, return_state=True)
     s = HipSolver(I.starmap_init(prob), precision=precision)
     try:
