@@ -226,15 +226,16 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
                                              0);
         }
     };
-    auto src_dma = [&](int buf) {
+    // the source copies of a group whose nl load records are at ld (in the
+    // LDS ring, or in the plan in global memory)
+    auto src_dma_recs = [&](const int nl, const auto *ld) {
         // one load record per lane, read before the first copy is issued (an
         // LDS read behind an outstanding copy would have to wait for it):
         // comp | slot << 20 | (origin-shift bits) << 28
-        const int nl = __builtin_amdgcn_readfirstlane(hdr_nl(buf));
         const int cw = (tid >> 6) - CW;  // copy wave cw: every NCW-th load
         int myrec = 0;
         if (lane < nl) {
-            const DevLoad &d = ldrec(buf)[lane];
+            const auto &d = ld[lane];
             myrec = d.comp | (d.slot << 20) |
                     (((d.org[0] & 1) | ((d.org[1] & 1) << 1) | ((d.org[2] & 1) << 2)) << 28);
         }
@@ -270,6 +271,17 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
                 }
             }
         }
+    };
+
+    auto src_dma = [&](int buf) { src_dma_recs(__builtin_amdgcn_readfirstlane(hdr_nl(buf)), ldrec(buf)); };
+    // group gq's copies from its load records in the plan itself: the first
+    // group of a tile needs no LDS blob, so its copies start in the
+    // prologue's first phase
+    auto src_dma_global = [&](int gq) {
+        const SMP_CONST unsigned char *gb = (const SMP_CONST unsigned char *)a.blobs + (int64_t)gq * BLOB_SZ;
+        src_dma_recs(*(const SMP_CONST int *)gb,
+                     reinterpret_cast<SMP_GLOBAL const DevLoad *>(glb(a.blobs) + (int64_t)gq * BLOB_SZ +
+                                                                  BLOB_LD_OFF));
     };
 
     // ---- cell waves ------------------------------------------------------
@@ -454,18 +466,26 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
     };
 
     // ---- copy waves ----------------------------------------------------------
-    // prologue (two barriers), then one barrier per group: the sources of
-    // group g+1 and the records of group g+2 while the cells update group g
+    // prologue (one barrier: group 0's sources from its records in global
+    // memory, beside the records of groups 0 and 1), then one barrier per
+    // group: the sources of group g+1 and the records of group g+2 while the
+    // cells update group g
     if (copier) {
         if (ng > 0) blob_dma(0, 0);
         if (ng > 1) blob_dma(1, 1);
         if (warmer && ng > 0) warm_records(0);
         if (warmer && ng > 1) warm_records(1);
+#if SMP_SCALAR_RECORDS
+        if (ng > 0) src_dma_global(0);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+#else
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         if (ng > 0) src_dma(0);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
+#endif
         for (int gq = 0; gq < ng; ++gq) {
             if (gq + 1 < ng) src_dma((gq + 1) % 3);
             if (gq + 2 < ng) blob_dma(gq + 2, (gq + 2) % 3);
@@ -501,9 +521,15 @@ __device__ __forceinline__ void substep_tile(const AT &a, const int tile, unsign
             }
     }
     for (int e = tid; e < S; e += NCT) lds[e] = (T)0;   // slot 0: the zero tile
+#if SMP_SCALAR_RECORDS
+    // the records come from global memory: no barrier before the first loads
+    if (ng > 0) load_targets(0, 0, uA);
+    __syncthreads();
+#else
     __syncthreads();
     if (ng > 0) load_targets(0, 0, uA);
     __syncthreads();
+#endif
     for (int gq = 0; gq < ng; gq += 2) {
         update(gq, gq % 3, uA);
         if (gq + 1 < ng) load_targets(gq + 1, (gq + 1) % 3, uB);
