@@ -644,20 +644,26 @@ hipError_t launch_substep(const SubstepArgs<T> &a, int nt, bool gen, int cw, siz
     if (a.ncf > MAX_CFIELDS || (cw != 4 && cw != 8) || a.chunks > CPS_MAX ||
         (a.napp != 1 && a.napp != 2) || (a.pair && !pair_built<T>(cw)))
         return hipErrorInvalidValue;
+    // terms per record: 8 (2D P_N: up to four per axis), 10 (3D P7), 12
     if (a.pair) {
         if constexpr (sizeof(T) == 4) {
             if (cw == 8)
-                return nt <= 8 ? launch_nt<T, 8, 8, true>(a, gen, lds_bytes, s)
-                               : launch_nt<T, MAXT, 8, true>(a, gen, lds_bytes, s);
-            return nt <= 8 ? launch_nt<T, 8, 4, true>(a, gen, lds_bytes, s)
-                           : launch_nt<T, MAXT, 4, true>(a, gen, lds_bytes, s);
+                return nt <= 8    ? launch_nt<T, 8, 8, true>(a, gen, lds_bytes, s)
+                       : nt <= 10 ? launch_nt<T, 10, 8, true>(a, gen, lds_bytes, s)
+                                  : launch_nt<T, MAXT, 8, true>(a, gen, lds_bytes, s);
+            return nt <= 8    ? launch_nt<T, 8, 4, true>(a, gen, lds_bytes, s)
+                   : nt <= 10 ? launch_nt<T, 10, 4, true>(a, gen, lds_bytes, s)
+                              : launch_nt<T, MAXT, 4, true>(a, gen, lds_bytes, s);
         }
         return hipErrorInvalidValue;
     }
-    if (cw == 8) return nt <= 8 ? launch_nt<T, 8, 8, false>(a, gen, lds_bytes, s)
-                                : launch_nt<T, MAXT, 8, false>(a, gen, lds_bytes, s);
-    return nt <= 8 ? launch_nt<T, 8, 4, false>(a, gen, lds_bytes, s)
-                   : launch_nt<T, MAXT, 4, false>(a, gen, lds_bytes, s);
+    if (cw == 8)
+        return nt <= 8    ? launch_nt<T, 8, 8, false>(a, gen, lds_bytes, s)
+               : nt <= 10 ? launch_nt<T, 10, 8, false>(a, gen, lds_bytes, s)
+                          : launch_nt<T, MAXT, 8, false>(a, gen, lds_bytes, s);
+    return nt <= 8    ? launch_nt<T, 8, 4, false>(a, gen, lds_bytes, s)
+           : nt <= 10 ? launch_nt<T, 10, 4, false>(a, gen, lds_bytes, s)
+                      : launch_nt<T, MAXT, 4, false>(a, gen, lds_bytes, s);
 }
 
 // (the dynamic LDS limit of batch_kernel is raised by batch_occupancy, which
@@ -667,6 +673,9 @@ static const void *batch_fn_cw(int nt, bool gen) {
     if (nt <= 8)
         return gen ? (const void *)batch_kernel<T, 8, NAPP, true, CW, PAIR>
                    : (const void *)batch_kernel<T, 8, NAPP, false, CW, PAIR>;
+    if (nt <= 10)
+        return gen ? (const void *)batch_kernel<T, 10, NAPP, true, CW, PAIR>
+                   : (const void *)batch_kernel<T, 10, NAPP, false, CW, PAIR>;
     return gen ? (const void *)batch_kernel<T, MAXT, NAPP, true, CW, PAIR>
                : (const void *)batch_kernel<T, MAXT, NAPP, false, CW, PAIR>;
 }
