@@ -85,8 +85,10 @@ def test_fp32_batch_within_1e5_of_fp64_oracle(n, orders):
         b.run()
         for i, (p, s) in enumerate(zip(pars, b.solvers)):
             print("oracle", i, p["n"][1], p["n_mom"], flush=True)
-            ref, rpar = so.starmap_solver(dict(p), verbose=False, return_state=True)
+            # every moment field, whatever the problem's own mom_output
+            ref, rpar = so.starmap_solver(dict(p, mom_output="all"), verbose=False, return_state=True)
             assert rpar["_oracle"]["step_count"] == b.n_steps[i]
+            assert len(ref) == s.n_sys
             got = [s.get_state(j).reshape(np.shape(r["U"]), order="F") for j, r in enumerate(ref)]
             e = check_fp32_bound(got, [r["U"] for r in ref])
             print("  fp32 errors", e, flush=True)
