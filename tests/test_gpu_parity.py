@@ -101,7 +101,16 @@ CASES = {
     "lattice2d_moving_filter_full": lambda: dict(
         P.lattice_2d(n=36, n_mom=4, tfinal=0.4, f_position="full"),
         filterfunction=P.exp_filter_moving(2, 5.0)),
+    # a filter that varies in space (starmap_solver.m:284-305 with x, y): the
+    # kernel reads the filter fields from global memory (its variant with
+    # global-memory coefficients)
+    "lattice2d_filter_field": lambda: dict(P.lattice_2d(n=50, n_mom=7, tfinal=0.3),
+                                           filterfunction=spatial_filter),
 }
+
+
+def spatial_filter(par, dt, k, x, y):
+    return np.exp(-dt * 8.0 * (1 + 0.1 * x) * (k / par["n_mom"]) ** 4)
 
 
 @pytest.mark.parametrize("case", sorted(CASES))
@@ -120,7 +129,7 @@ def test_3d_extrapolation_filter():
 
 
 @pytest.mark.parametrize("case", ["gauss2d_p5_100", "lattice2d_p7_70", "lattice2d_p39_48",
-                                  "gauss3d_p7_20"])
+                                  "gauss3d_p7_20", "lattice2d_filter_field"])
 def test_fp32_within_1e5(case):
     """The FP32 state against the FP64 oracle, within the bound of
     tests/fp32_bounds.py."""
