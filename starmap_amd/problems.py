@@ -190,6 +190,45 @@ def boxes_2d(n=50, n_mom=9, closure="P", tfinal=0.2):
     }
 
 
+def gauss_1d(n=1000, n_mom=19, closure="P", tfinal=0.4, n_plot=21):
+    """starmap_1d_ex_gauss.m:32-65: 1D P19 on [0,1] with 1000 cells,
+    periodic, zeroth moment exp(-500 (x-1/2)^2), sigma_a = sigma_s0 = 1.5
+    (constant handles returning scalars), outputs at 21 times up to 0.4."""
+    return {
+        "name": "1D Gaussian",
+        "closure": closure,
+        "n_mom": n_mom,
+        "n": [n],
+        "sigma_a": lambda x: 1.5,
+        "sigma_s0": lambda x: 1.5,
+        "ic": lambda x, k: np.exp(-500 * (x - 0.5) ** 2) * (k == 1),
+        "ax": [0.0, 1.0],
+        "bc": [0],
+        "t_plot": list(np.linspace(0, tfinal, n_plot)),
+        "output": None,
+    }
+
+
+def planesource_1d(n=1000, n_mom=7, closure="SP", tfinal=0.4, n_plot=21):
+    """starmap_1d_ex_planesource.m:32-67: 1D SP7 on [-0.6,0.6] with 1000
+    cells, periodic, the plane source smoothed to a Gaussian of pseudo-time
+    3.2e-3, pure scattering (sigma_a = 0, sigma_s0 = 1)."""
+    t = 3.2e-3
+    return {
+        "name": "Plane Source Test",
+        "closure": closure,
+        "n_mom": n_mom,
+        "n": [n],
+        "sigma_a": lambda x: 0.0,
+        "sigma_s0": lambda x: 1.0,
+        "ic": lambda x, k: 1 / (4 * np.pi * t) * np.exp(-(x ** 2) / (4 * t)) * (k == 1),
+        "ax": [-0.6, 0.6],
+        "bc": [0],
+        "t_plot": list(np.linspace(0, tfinal, n_plot)),
+        "output": None,
+    }
+
+
 def controlrod_2d(n=251, n_mom=3, closure="SP", tfinal=0.9, n_plot=46):
     """starmap_2d_ex_controlrod.m:35-81: a control rod moves in and out of a
     multiplying medium.  Effective absorption rod(x,y)*s(t), with s rising

@@ -57,7 +57,8 @@ def test_product_closure_matches_oracle_bitwise(clo, dim):
 def test_init_lifting_matches_oracle():
     from starmap_amd import problems as P
     for prob in (P.gauss_periodic_2d(n=10), P.lattice_2d(n=14, n_mom=5),
-                 P.gauss_periodic_3d(n=6, n_mom=3), P.boxes_2d(n=10, n_mom=3)):
+                 P.gauss_periodic_3d(n=6, n_mom=3), P.boxes_2d(n=10, n_mom=3),
+                 P.gauss_1d(n=20), P.planesource_1d(n=20)):
         a = I.starmap_init(prob)
         b = so.starmap_init(prob)
         for k in ("dim", "ax", "n", "bc", "source_ind", "tfinal", "cn", "closure_mod"):

@@ -106,6 +106,11 @@ CASES = {
     # global-memory coefficients)
     "lattice2d_filter_field": lambda: dict(P.lattice_2d(n=50, n_mom=7, tfinal=0.3),
                                            filterfunction=spatial_filter),
+    # the 1D examples at full size (starmap_1d_ex_gauss.m, P19 1000 cells;
+    # starmap_1d_ex_planesource.m, SP7 1000 cells): 1D problems lifted to a
+    # 1 x 1 x n grid, one output time each
+    "gauss1d_p19_1000": lambda: P.gauss_1d(),
+    "planesource1d_sp7_1000": lambda: P.planesource_1d(),
 }
 
 
@@ -129,7 +134,8 @@ def test_3d_extrapolation_filter():
 
 
 @pytest.mark.parametrize("case", ["gauss2d_p5_100", "lattice2d_p7_70", "lattice2d_p39_48",
-                                  "gauss3d_p7_20", "lattice2d_filter_field"])
+                                  "gauss3d_p7_20", "lattice2d_filter_field",
+                                  "gauss1d_p19_1000", "planesource1d_sp7_1000"])
 def test_fp32_within_1e5(case):
     """The FP32 state against the FP64 oracle, within the bound of
     tests/fp32_bounds.py."""
