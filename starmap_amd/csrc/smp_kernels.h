@@ -21,13 +21,18 @@ constexpr int CPS_MAX = 16;   // DMA chunks per LDS slot
 // Waves copying sources into LDS beside cw cell waves (4 or 8).  The copies
 // are issued 256 bytes per instruction, so more copy waves keep more bytes
 // in flight: 8 beside 8 cell waves (one workgroup of 16 waves per CU), 4
-// beside 4 (two workgroups of 8 waves per CU).  The batched kernel keeps 4
-// in both cases (with 8 cell waves its FP64 variants need more than the 128
-// VGPRs a 16-wave workgroup leaves).
+// beside 4 (two workgroups of 8 waves per CU).  The batched kernel has 8
+// beside 8 FP32 cell waves with records of up to 10 terms (1% faster on the
+// 64-problem sweep than 4), and 4 otherwise: its FP64 variants and those
+// with 12-term records need more than the 128 VGPRs a 16-wave workgroup
+// leaves.  nt: the record's term count (the template's NT, or the run-time
+// count it is chosen by: <= 8, <= 10, more).
 constexpr int copy_waves(int cw) { return cw == 8 ? 8 : 4; }
-constexpr int batch_copy_waves(int) { return 4; }
+template <typename T>
+constexpr int batch_copy_waves(int cw, int nt) { return sizeof(T) == 4 && cw == 8 && nt <= 10 ? 8 : 4; }
 constexpr int kthreads(int cw) { return (cw + copy_waves(cw)) * 64; }
-constexpr int batch_threads(int cw) { return (cw + batch_copy_waves(cw)) * 64; }
+template <typename T>
+constexpr int batch_threads(int cw, int nt) { return (cw + batch_copy_waves<T>(cw, nt)) * 64; }
 
 template <typename T>
 struct SubstepArgs {

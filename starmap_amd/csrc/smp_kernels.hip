@@ -554,7 +554,7 @@ __global__ __launch_bounds__(kthreads(CW)) void substep_kernel(const SubstepArgs
 // tile, heaviest first; every workgroup takes the next unit from a counter
 // until none is left, so uneven problems balance over the CUs.
 template <typename T, int NT, int NAPP, bool GEN, int CW, bool PAIR>
-__global__ __launch_bounds__(batch_threads(CW)) void batch_kernel(const BatchLaunch<T> *bl) {
+__global__ __launch_bounds__(batch_threads<T>(CW, NT)) void batch_kernel(const BatchLaunch<T> *bl) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int s_unit;
     const int n_units = bl->n_units;
@@ -591,8 +591,8 @@ __global__ __launch_bounds__(batch_threads(CW)) void batch_kernel(const BatchLau
         // the launch's application count is compiled in (a runtime count
         // costs the update its instruction schedule, and both tile paths in
         // one kernel outgrow the instruction cache)
-        substep_tile<T, NT, NAPP, GEN, CW, batch_copy_waves(CW), true, PAIR, false>(a, u - ent[lo].unit0,
-                                                                                      smem_raw);
+        substep_tile<T, NT, NAPP, GEN, CW, batch_copy_waves<T>(CW, NT), true, PAIR, false>(
+            a, u - ent[lo].unit0, smem_raw);
     }
 }
 
@@ -704,7 +704,7 @@ hipError_t launch_batch(const BatchLaunch<T> *bl, int nt, bool gen, int cw, int 
     const void *f = batch_fn<T>(nt, gen, cw, napp, pair);
     if (!f) return hipErrorInvalidValue;
     void *args[] = {(void *)&bl};
-    return hipLaunchKernel(f, dim3(grid), dim3(batch_threads(cw)), args, lds_bytes, s);
+    return hipLaunchKernel(f, dim3(grid), dim3(batch_threads<T>(cw, nt)), args, lds_bytes, s);
 }
 
 template <typename T>
@@ -716,7 +716,7 @@ hipError_t batch_occupancy(int nt, bool gen, int cw, int napp, bool pair, size_t
     if (!f) return hipErrorInvalidValue;
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 64);
     if (e != hipSuccess) return e;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, f, batch_threads(cw), lds_bytes);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, f, batch_threads<T>(cw, nt), lds_bytes);
 }
 
 hipError_t launch_fill(int *p, int n, int v, hipStream_t s) {
