@@ -166,9 +166,12 @@ int smp_observables(smp_problem *h, double *sums, double *sq, int32_t *nonfinite
 
 /* Timing of the last smp_step call: elapsed milliseconds of all its kernels
  * measured with HIP events on the handle's stream; the time of the substep
- * kernels alone (sum over the timed launches), their launch count and their
- * algorithmic HBM bytes (each staged source component read once, each
- * target component read and written once). */
+ * kernels alone, their launch count and their algorithmic HBM bytes (each
+ * staged source component read once, each target component read and written
+ * once).  The substep launches run back to back, and their time is the
+ * elapsed time; with the 'full' filter, whose scale kernels run between
+ * them, it is the sum of an event pair around each substep launch (the
+ * first 12288). */
 int smp_last_step_timing(const smp_problem *h, double *total_ms, double *substep_ms,
                          int64_t *substep_launches, double *substep_bytes);
 

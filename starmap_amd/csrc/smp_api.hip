@@ -1002,10 +1002,15 @@ int run_steps(smp_problem *h, double dt, int64_t n_steps) {
     }
     int64_t n_sub = 0;
     for (auto *p : seq) n_sub += p != nullptr;
-    // a start/end event pair around every substep launch (up to a cap):
-    // their differences are the substep kernel durations on this stream
-    const int64_t timed = std::min<int64_t>(n_sub, 3 * 4096);
-    while ((int64_t)h->events.size() < 2 * timed + 2) {
+    // Substep kernel time.  A sequence of substep launches only runs them
+    // back to back, and its elapsed time is theirs.  Where the full filter's
+    // scale kernels come between them, a start/end event pair around every
+    // substep launch (up to a cap) separates the substep kernels; each pair
+    // leaves the GPU idle for about 10 us between launches, so the pairs are
+    // recorded only there.
+    const bool per_launch = full;
+    const int64_t timed = per_launch ? std::min<int64_t>(n_sub, 3 * 4096) : n_sub;
+    while ((int64_t)h->events.size() < (per_launch ? 2 * timed + 2 : 2)) {
         hipEvent_t e;
         HIP_TRY(hipEventCreate(&e));
         h->events.push_back(e);
@@ -1024,10 +1029,10 @@ int run_steps(smp_problem *h, double dt, int64_t n_steps) {
             continue;
         }
         const int kind = p == &even ? 1 : 0;
-        if (li < timed) HIP_TRY(hipEventRecord(h->events[2 + 2 * li], h->stream));
+        if (per_launch && li < timed) HIP_TRY(hipEventRecord(h->events[2 + 2 * li], h->stream));
         HIP_TRY(smp::launch_substep<T>(*p, h->plan[kind].max_terms, h->gen[kind], h->cell_waves,
                                        h->lds_bytes[kind], h->stream));
-        if (li < timed) HIP_TRY(hipEventRecord(h->events[3 + 2 * li], h->stream));
+        if (per_launch && li < timed) HIP_TRY(hipEventRecord(h->events[3 + 2 * li], h->stream));
         ++li;
     }
     HIP_TRY(hipEventRecord(ev_end, h->stream));
@@ -1035,8 +1040,8 @@ int run_steps(smp_problem *h, double dt, int64_t n_steps) {
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, ev_start, ev_end));
     h->last_total_ms = ms;
-    double sub_ms = 0;
-    for (int64_t i = 0; i < timed; ++i) {
+    double sub_ms = per_launch ? 0.0 : (double)ms;
+    for (int64_t i = 0; per_launch && i < timed; ++i) {
         float m = 0;
         HIP_TRY(hipEventElapsedTime(&m, h->events[2 + 2 * i], h->events[3 + 2 * i]));
         sub_ms += m;
