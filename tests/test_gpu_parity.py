@@ -247,6 +247,38 @@ def test_fused_steps_bitwise_equal_single_steps(precision):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("f_position", ["substep", "full"])
+def test_step_timing(f_position):
+    """smp_last_step_timing after one call: the substep launch count of the
+    call (2n+1 for n fused steps, 3n with the 'full' filter), their time
+    within the call's elapsed time (equal to it when only substep kernels
+    run; with the 'full' filter the scale kernels run between them), and
+    their algorithmic bytes (each source component read once, each target
+    read and written once per launch)."""
+    prob = P.lattice_2d(n=45, n_mom=9, tfinal=1.0, f_position=f_position)
+    s = HipSolver(I.starmap_init(prob), precision="fp32")
+    try:
+        dt = s.prepare()
+        n = 5
+        s.step(dt, n)
+        tot, sub, launches, by = s.timing()
+        n_even = int(np.sum((s.gt - 1).sum(axis=1) % 2 == 0))
+        n_odd = s.n_sys - n_even
+        cells = int(np.prod(s.n))
+        if f_position == "full":
+            assert launches == 3 * n
+            assert 0 < sub < tot
+            kinds = [0, 1, 0] * n
+        else:
+            assert launches == 2 * n + 1
+            assert 0 < sub == tot
+            kinds = [0] + [1, 0] * n
+        want = sum(4 * cells * (n_even + 2 * n_odd if k == 0 else n_odd + 2 * n_even) for k in kinds)
+        assert by == want
+    finally:
+        s.close()
+
+
 def test_summary_print_matches_oracle(capsys):
     """The overview lines of starmap_solver.m:477-482 are the oracle's, and
     the CPU-time lines have the reference's format."""
