@@ -111,7 +111,18 @@ CASES = {
     # 1 x 1 x n grid, one output time each
     "gauss1d_p19_1000": lambda: P.gauss_1d(),
     "planesource1d_sp7_1000": lambda: P.planesource_1d(),
+    # SP_N in 3D (the 3D SP_N closure's coupling)
+    "gauss3d_sp5_12": lambda: P.gauss_periodic_3d(n=12, n_mom=5, closure="SP", tfinal=0.6, std=0.2),
+    # periodic in x, extrapolation in y, on a domain the Gaussian's tails
+    # reach; off centre, or the periodic neighbour of an edge cell would
+    # equal the cell itself and both conditions would give the same result
+    "gauss2d_mixed_bc": lambda: dict(P.gauss_periodic_2d(n=40, n_mom=5, tfinal=0.6, half=0.5),
+                                     bc=[0, 1], ic=_offset_gaussian),
 }
+
+
+def _offset_gaussian(x, y):
+    return np.exp(-((x - 0.2) ** 2 + (y + 0.1) ** 2) / (2 * 0.2 ** 2))
 
 
 def spatial_filter(par, dt, k, x, y):
