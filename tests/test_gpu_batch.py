@@ -77,6 +77,32 @@ def test_batch_bitwise_equal_single_handles(precision, cw, use_graph):
         b.close()
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_batch_of_2d_and_3d_problems_bitwise_equal_single_handles(precision):
+    """2D and 3D problems in one batch: the launch takes the records of the
+    problem with the most terms (10 for 3D P7, against 8 for 2D P_N), so the
+    2D problems run the wider kernel, whose extra terms add exact zeros; each
+    problem equals its single-handle run bit for bit, as does a 3D problem
+    with a non-cubic grid and extrapolation boundaries."""
+    probs = [P.lattice_2d(n=40, n_mom=9, tfinal=0.2),
+             P.gauss_periodic_3d(n=14, n_mom=7, tfinal=0.2, std=0.2),
+             dict(P.gauss_periodic_3d(n=11, n_mom=3, tfinal=0.2, std=0.2), n=[11, 9, 7], bc=[1, 1, 1]),
+             P.batch_instance(48, 15, 5.0, np.full((4, 4), 0.5), np.full((4, 4), 2.0), tfinal=0.2)]
+    pars = [I.starmap_init(p) for p in probs]
+    print("batch 2d+3d", precision, [tuple(p["n"]) for p in pars], flush=True)
+    ref = single_runs(pars, precision)
+    b = BatchSolver(pars, precision=precision)
+    try:
+        for use_graph in (False, True):
+            dts, ns = b.prepare()            # from the initial state again
+            b.step(ns, use_graph=use_graph)
+            for i, s in enumerate(b.solvers):
+                for j, (x, y) in enumerate(zip(states(s), ref[i])):
+                    assert np.array_equal(x, y), (use_graph, i, j)
+    finally:
+        b.close()
+
+
 @pytest.mark.parametrize("n,orders", [(4, (7, 3, 5)), (2, (31, 39))])
 def test_fp32_batch_within_1e5_of_fp64_oracle(n, orders):
     pars = small_sweep(n=n, orders=orders, tfinal=0.2)
